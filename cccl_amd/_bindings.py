@@ -1,0 +1,224 @@
+"""ctypes view of libcccl_amd.so: the cccl_device_radix_sort_* C-ABI
+(include/cccl/c/radix_sort.h) and the typed cccl_amd_* entry points
+(include/cccl_amd/radix_sort_amd.h).
+
+Struct layouts follow include/cccl/c/types.h field for field.  Loading fails loudly
+when the library has not been built: there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libcccl_amd.so"
+
+# cccl_type_enum
+CCCL_INT8, CCCL_INT16, CCCL_INT32, CCCL_INT64 = 0, 1, 2, 3
+CCCL_UINT8, CCCL_UINT16, CCCL_UINT32, CCCL_UINT64 = 4, 5, 6, 7
+CCCL_FLOAT16, CCCL_FLOAT32, CCCL_FLOAT64, CCCL_STORAGE, CCCL_BOOLEAN = 8, 9, 10, 11, 12
+# cccl_sort_order_t
+CCCL_ASCENDING, CCCL_DESCENDING = 0, 1
+# cccl_iterator_kind_t
+CCCL_POINTER, CCCL_ITERATOR = 0, 1
+# cccl_op_kind_t (subset)
+CCCL_STATELESS, CCCL_IDENTITY = 0, 20
+
+CUDA_SUCCESS = 0
+CUDA_ERROR_INVALID_VALUE = 1
+CUDA_ERROR_OUT_OF_MEMORY = 2
+CUDA_ERROR_NOT_SUPPORTED = 801
+CUDA_ERROR_UNKNOWN = 999
+
+
+class cccl_type_info(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_size_t), ("alignment", ctypes.c_size_t), ("type", ctypes.c_int)]
+
+
+class cccl_op_t(ctypes.Structure):
+    _fields_ = [
+        ("type", ctypes.c_int),
+        ("name", ctypes.c_char_p),
+        ("code", ctypes.c_char_p),
+        ("code_size", ctypes.c_size_t),
+        ("code_type", ctypes.c_int),
+        ("size", ctypes.c_size_t),
+        ("alignment", ctypes.c_size_t),
+        ("state", ctypes.c_void_p),
+        ("extra_ltoirs", ctypes.c_void_p),
+        ("extra_ltoir_sizes", ctypes.c_void_p),
+        ("num_extra_ltoirs", ctypes.c_size_t),
+        ("extra_code_types", ctypes.c_void_p),
+    ]
+
+
+class cccl_build_config(ctypes.Structure):
+    _fields_ = [
+        ("extra_compile_flags", ctypes.c_void_p),
+        ("num_extra_compile_flags", ctypes.c_size_t),
+        ("extra_include_dirs", ctypes.c_void_p),
+        ("num_extra_include_dirs", ctypes.c_size_t),
+    ]
+
+
+class cccl_iterator_t(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_size_t),
+        ("alignment", ctypes.c_size_t),
+        ("type", ctypes.c_int),
+        ("advance", cccl_op_t),
+        ("dereference", cccl_op_t),
+        ("value_type", cccl_type_info),
+        ("state", ctypes.c_void_p),
+        ("host_advance", ctypes.c_void_p),
+    ]
+
+
+_NAMES = [
+    "single_tile_kernel", "upsweep_kernel", "alt_upsweep_kernel", "scan_bins_kernel", "downsweep_kernel",
+    "alt_downsweep_kernel", "histogram_kernel", "exclusive_sum_kernel", "init_bins_and_counters_kernel",
+    "init_lookback_kernel", "onesweep_kernel",
+]
+
+
+class cccl_device_radix_sort_build_result_t(ctypes.Structure):
+    _fields_ = (
+        [
+            ("cc", ctypes.c_int),
+            ("payload", ctypes.c_void_p),
+            ("payload_size", ctypes.c_size_t),
+            ("payload_kind", ctypes.c_int),
+            ("library", ctypes.c_void_p),
+            ("key_type", cccl_type_info),
+            ("value_type", cccl_type_info),
+        ]
+        + [(n, ctypes.c_void_p) for n in _NAMES]
+        + [("order", ctypes.c_int), ("runtime_policy", ctypes.c_void_p), ("runtime_policy_size", ctypes.c_size_t)]
+        + [(n + "_lowered_name", ctypes.c_char_p) for n in _NAMES]
+    )
+
+
+class radix_sort_config(ctypes.Structure):
+    """cccl_amd_radix_sort_config_t"""
+    _fields_ = [
+        ("onesweep_threads", ctypes.c_int32),
+        ("onesweep_ipt", ctypes.c_int32),
+        ("radix_bits", ctypes.c_int32),
+        ("single_tile_threads", ctypes.c_int32),
+        ("single_tile_ipt", ctypes.c_int32),
+        ("histogram_blocks_per_cu", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
+    ]
+
+    def as_tuple(self):
+        return (self.onesweep_threads, self.onesweep_ipt, self.radix_bits, self.single_tile_threads,
+                self.single_tile_ipt, self.histogram_blocks_per_cu)
+
+
+class segmented_config(ctypes.Structure):
+    """cccl_amd_segmented_config_t"""
+    _fields_ = [
+        ("num_block_classes", ctypes.c_int32),
+        ("block_threads", ctypes.c_int32 * 4),
+        ("block_ipt", ctypes.c_int32 * 4),
+        ("large_threads", ctypes.c_int32),
+        ("large_ipt", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
+    ]
+
+
+# Every symbol the headers under include/ declare for this library.
+EXPORTED_SYMBOLS = [
+    "cccl_device_radix_sort_build",
+    "cccl_device_radix_sort_build_ex",
+    "cccl_device_radix_sort_compile",
+    "cccl_device_radix_sort_load",
+    "cccl_device_radix_sort",
+    "cccl_device_radix_sort_link_ltoir",
+    "cccl_device_radix_sort_serialize",
+    "cccl_device_radix_sort_deserialize",
+    "cccl_device_radix_sort_cleanup",
+    "cccl_serialization_buffer_free",
+    "cccl_amd_radix_sort",
+    "cccl_amd_segmented_radix_sort",
+    "cccl_amd_radix_sort_default_config",
+    "cccl_amd_segmented_default_config",
+    "cccl_amd_radix_sort_config_valid",
+    "cccl_amd_onesweep_shapes",
+    "cccl_amd_block_sort_shapes",
+    "cccl_amd_profile_enable",
+    "cccl_amd_profile_read",
+]
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m cccl_amd.build` (the sort has no CPU fallback)")
+    L = ctypes.CDLL(str(LIB_PATH))
+    vp, sz, i32, u64, i64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64, ctypes.c_int64
+    BR = cccl_device_radix_sort_build_result_t
+    build_args = [ctypes.POINTER(BR), i32, cccl_iterator_t, cccl_iterator_t, cccl_op_t, ctypes.c_char_p, i32, i32,
+                  ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
+    L.cccl_device_radix_sort_build.argtypes = build_args
+    L.cccl_device_radix_sort_build_ex.argtypes = build_args + [ctypes.POINTER(cccl_build_config)]
+    L.cccl_device_radix_sort_compile.argtypes = build_args + [ctypes.POINTER(cccl_build_config)]
+    L.cccl_device_radix_sort_load.argtypes = [ctypes.POINTER(BR)]
+    L.cccl_device_radix_sort.argtypes = [BR, vp, ctypes.POINTER(sz), cccl_iterator_t, cccl_iterator_t,
+                                         cccl_iterator_t, cccl_iterator_t, cccl_op_t, u64, i32, i32, ctypes.c_bool,
+                                         ctypes.POINTER(i32), vp]
+    L.cccl_device_radix_sort_link_ltoir.argtypes = [ctypes.POINTER(BR), vp, vp, sz]
+    L.cccl_device_radix_sort_serialize.argtypes = [ctypes.POINTER(BR), ctypes.POINTER(vp), ctypes.POINTER(sz)]
+    L.cccl_device_radix_sort_deserialize.argtypes = [ctypes.POINTER(BR), vp, sz]
+    L.cccl_device_radix_sort_cleanup.argtypes = [ctypes.POINTER(BR)]
+    for name in ("cccl_device_radix_sort_build", "cccl_device_radix_sort_build_ex", "cccl_device_radix_sort_compile",
+                 "cccl_device_radix_sort_load", "cccl_device_radix_sort", "cccl_device_radix_sort_link_ltoir",
+                 "cccl_device_radix_sort_serialize", "cccl_device_radix_sort_deserialize",
+                 "cccl_device_radix_sort_cleanup"):
+        getattr(L, name).restype = ctypes.c_int
+    L.cccl_serialization_buffer_free.argtypes = [vp]
+    L.cccl_serialization_buffer_free.restype = None
+    L.cccl_amd_radix_sort.argtypes = [vp, ctypes.POINTER(sz), vp, vp, vp, vp, ctypes.POINTER(i32), i32, sz, u64,
+                                      i32, i32, i32, i32, ctypes.POINTER(radix_sort_config), vp]
+    L.cccl_amd_radix_sort.restype = i32
+    L.cccl_amd_segmented_radix_sort.argtypes = [vp, ctypes.POINTER(sz), vp, vp, vp, vp, ctypes.POINTER(i32), i32,
+                                                sz, u64, i64, vp, vp, i32, i32, i32, i32, i32,
+                                                ctypes.POINTER(segmented_config), vp]
+    L.cccl_amd_segmented_radix_sort.restype = i32
+    L.cccl_amd_radix_sort_default_config.argtypes = [i32, i32, u64, ctypes.POINTER(radix_sort_config)]
+    L.cccl_amd_radix_sort_default_config.restype = None
+    L.cccl_amd_segmented_default_config.argtypes = [i32, i32, ctypes.POINTER(segmented_config)]
+    L.cccl_amd_segmented_default_config.restype = None
+    L.cccl_amd_radix_sort_config_valid.argtypes = [ctypes.POINTER(radix_sort_config), i32, i32]
+    L.cccl_amd_radix_sort_config_valid.restype = i32
+    L.cccl_amd_onesweep_shapes.argtypes = [ctypes.POINTER(i32), ctypes.POINTER(i32), i32]
+    L.cccl_amd_onesweep_shapes.restype = i32
+    L.cccl_amd_block_sort_shapes.argtypes = [ctypes.POINTER(i32), ctypes.POINTER(i32), i32]
+    L.cccl_amd_block_sort_shapes.restype = i32
+    L.cccl_amd_profile_enable.argtypes = [i32]
+    L.cccl_amd_profile_enable.restype = None
+    L.cccl_amd_profile_read.argtypes = [ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_float), i32]
+    L.cccl_amd_profile_read.restype = i32
+    _lib = L
+    return L
+
+
+def pointer_iterator(value_type: int, size: int, ptr: int | None) -> cccl_iterator_t:
+    """A CCCL_POINTER iterator over device memory (state = the device address)."""
+    it = cccl_iterator_t()
+    it.size = 8
+    it.alignment = 8
+    it.type = CCCL_POINTER
+    it.value_type = cccl_type_info(size, max(size, 1), value_type)
+    it.state = ptr
+    return it
+
+
+def identity_op() -> cccl_op_t:
+    return cccl_op_t()

@@ -1,0 +1,81 @@
+"""Builds libcccl_amd.so (gfx950 code objects + host dispatch + C-ABI) in-tree.
+
+Usage: python -m cccl_amd.build  (or build() from Python).  Objects go to
+<repo>/_build/, the shared library to cccl_amd/libcccl_amd.so.
+"""
+
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+REPO = PKG.parent
+CSRC = PKG / "csrc"
+BUILD = REPO / "_build"
+LIB = PKG / "libcccl_amd.so"
+
+SOURCES = [
+    "inst_k1.hip",
+    "inst_k2.hip",
+    "inst_k4.hip",
+    "inst_k8.hip",
+    "dispatch.hip",
+    "segmented.hip",
+    "capi.hip",
+]
+
+# -packed-fp32-ops: no v_pk_*_f32 generation anywhere in the device code.
+HIP_FLAGS = [
+    "-std=c++17",
+    "-O3",
+    "--offload-arch=gfx950",
+    "-fPIC",
+    "-fvisibility=hidden",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found: the gfx950 build needs ROCm's hipcc")
+
+
+def _compile(src: str) -> Path:
+    obj = BUILD / (Path(src).stem + ".o")
+    cmd = [_hipcc(), *HIP_FLAGS, "-I", str(REPO / "include"), "-c", str(CSRC / src), "-o", str(obj)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(jobs: int | None = None, verbose: bool = False) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    jobs = jobs or min(len(SOURCES), os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,874 @@
+// The cccl_device_radix_sort_* C-ABI (c/parallel/include/cccl/c/radix_sort.h) and the
+// typed cccl_amd_* entry points, on top of the host dispatch.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+// The HIP runtime header maps CUDA_SUCCESS to hipSuccess; this file uses the C-ABI's
+// own CUresult enumerator of the same value instead.
+#undef CUDA_SUCCESS
+#define CCCL_C_EXPERIMENTAL
+#include "../../include/cccl/c/radix_sort.h"
+#include "../../include/cccl_amd/radix_sort_amd.h"
+#include "dispatch.hpp"
+#include "registry.hpp"
+
+namespace {
+
+using cccl_amd::DoubleBufferRef;
+using cccl_amd::SortProblem;
+
+// Key type -> (bytes, kind).  FLOAT16 covers any 16-bit IEEE-style float (fp16, bf16):
+// both transform alike (sign bit on top).
+bool key_traits(cccl_type_enum t, int& bytes, int& kind)
+{
+  switch (t)
+  {
+    case CCCL_INT8:
+      bytes = 1, kind = cccl_amd::kSignedKey;
+      return true;
+    case CCCL_INT16:
+      bytes = 2, kind = cccl_amd::kSignedKey;
+      return true;
+    case CCCL_INT32:
+      bytes = 4, kind = cccl_amd::kSignedKey;
+      return true;
+    case CCCL_INT64:
+      bytes = 8, kind = cccl_amd::kSignedKey;
+      return true;
+    case CCCL_UINT8:
+    case CCCL_BOOLEAN:
+      bytes = 1, kind = cccl_amd::kUnsignedKey;
+      return true;
+    case CCCL_UINT16:
+      bytes = 2, kind = cccl_amd::kUnsignedKey;
+      return true;
+    case CCCL_UINT32:
+      bytes = 4, kind = cccl_amd::kUnsignedKey;
+      return true;
+    case CCCL_UINT64:
+      bytes = 8, kind = cccl_amd::kUnsignedKey;
+      return true;
+    case CCCL_FLOAT16:
+      bytes = 2, kind = cccl_amd::kFloatKey;
+      return true;
+    case CCCL_FLOAT32:
+      bytes = 4, kind = cccl_amd::kFloatKey;
+      return true;
+    case CCCL_FLOAT64:
+      bytes = 8, kind = cccl_amd::kFloatKey;
+      return true;
+    default:
+      return false;
+  }
+}
+
+bool value_size_ok(size_t v)
+{
+  return v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16;
+}
+
+const char* width_name(int bytes)
+{
+  switch (bytes)
+  {
+    case 0:
+      return "none";
+    case 1:
+      return "u8";
+    case 2:
+      return "u16";
+    case 4:
+      return "u32";
+    case 8:
+      return "u64";
+    default:
+      return "u128";
+  }
+}
+
+// Build payload: which precompiled code objects a build result selects.
+struct BuildPayload
+{
+  char magic[8];
+  uint32_t version;
+  int32_t key_bytes;
+  int32_t key_kind;
+  int32_t value_bytes;
+  int32_t order;
+  char arch[20];
+};
+constexpr char kPayloadMagic[8] = {'C', 'C', 'A', 'M', 'D', 'C', 'O', '1'};
+
+// Handles returned by _load.  CUlibrary points at the library table, each CUkernel at
+// one of the kernel descriptors below.
+struct cccl_amd_library_st_impl
+{
+  const char* arch;
+};
+const cccl_amd_library_st_impl g_library{"gfx950"};
+struct KernelHandle
+{
+  const char* role;
+};
+const KernelHandle g_single_tile{"single_tile"};
+const KernelHandle g_histogram{"histogram"};
+const KernelHandle g_init{"init_bins_and_counters"};
+const KernelHandle g_onesweep{"onesweep"};
+
+char* dup_cstr(const std::string& s)
+{
+  char* p = new char[s.size() + 1];
+  std::memcpy(p, s.c_str(), s.size() + 1);
+  return p;
+}
+
+constexpr int kNumNames = 11;
+
+char** name_slot(cccl_device_radix_sort_build_result_t* b, int i)
+{
+  char** slots[kNumNames] = {
+    &b->single_tile_kernel_lowered_name,
+    &b->upsweep_kernel_lowered_name,
+    &b->alt_upsweep_kernel_lowered_name,
+    &b->scan_bins_kernel_lowered_name,
+    &b->downsweep_kernel_lowered_name,
+    &b->alt_downsweep_kernel_lowered_name,
+    &b->histogram_kernel_lowered_name,
+    &b->exclusive_sum_kernel_lowered_name,
+    &b->init_bins_and_counters_kernel_lowered_name,
+    &b->init_lookback_kernel_lowered_name,
+    &b->onesweep_kernel_lowered_name,
+  };
+  return slots[i];
+}
+
+void free_build(cccl_device_radix_sort_build_result_t* b)
+{
+  delete[] static_cast<char*>(b->payload);
+  std::free(b->runtime_policy);
+  for (int i = 0; i < kNumNames; ++i)
+  {
+    delete[] *name_slot(b, i);
+  }
+  std::memset(b, 0, sizeof(*b));
+}
+
+CUresult compile_impl(
+  cccl_device_radix_sort_build_result_t* build,
+  cccl_sort_order_t sort_order,
+  const cccl_iterator_t& input_keys_it,
+  const cccl_iterator_t& input_values_it,
+  const cccl_op_t& decomposer,
+  int cc_major,
+  int cc_minor)
+{
+  if (build == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  if (sort_order != CCCL_ASCENDING && sort_order != CCCL_DESCENDING)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  if (decomposer.name != nullptr && decomposer.name[0] != '\0')
+  {
+    return CUDA_ERROR_NOT_SUPPORTED; // custom decomposers need LTO-IR linking
+  }
+  int key_bytes = 0, key_kind = 0;
+  if (!key_traits(input_keys_it.value_type.type, key_bytes, key_kind)
+      || input_keys_it.value_type.size != size_t(key_bytes))
+  {
+    return CUDA_ERROR_NOT_SUPPORTED;
+  }
+  const bool keys_only = input_values_it.type == CCCL_POINTER && input_values_it.state == nullptr;
+  const size_t vbytes  = keys_only ? 0 : input_values_it.value_type.size;
+  if (!value_size_ok(vbytes))
+  {
+    return CUDA_ERROR_NOT_SUPPORTED;
+  }
+  if (!((cc_major == 0 && cc_minor == 0) || (cc_major == 9 && cc_minor == 5)))
+  {
+    return CUDA_ERROR_NOT_SUPPORTED; // only gfx950 code objects are built
+  }
+
+  auto* payload = new (std::nothrow) char[sizeof(BuildPayload)];
+  auto* policy  = static_cast<cccl_amd_radix_sort_config_t*>(std::malloc(sizeof(cccl_amd_radix_sort_config_t)));
+  if (payload == nullptr || policy == nullptr)
+  {
+    delete[] payload;
+    std::free(policy);
+    return CUDA_ERROR_OUT_OF_MEMORY;
+  }
+  BuildPayload bp{};
+  std::memcpy(bp.magic, kPayloadMagic, 8);
+  bp.version     = 1;
+  bp.key_bytes   = key_bytes;
+  bp.key_kind    = key_kind;
+  bp.value_bytes = int32_t(vbytes);
+  bp.order       = int32_t(sort_order);
+  std::snprintf(bp.arch, sizeof(bp.arch), "gfx950");
+  std::memcpy(payload, &bp, sizeof(bp));
+  // onesweep_threads == 0: pick the default shape per call from the item count.
+  std::memset(policy, 0, sizeof(*policy));
+  policy->radix_bits = 8;
+
+  cccl_device_radix_sort_build_result_t r{};
+  r.cc                  = 95;
+  r.payload             = payload;
+  r.payload_size        = sizeof(BuildPayload);
+  r.payload_kind        = CCCL_PAYLOAD_CUBIN;
+  r.key_type            = input_keys_it.value_type;
+  r.value_type          = keys_only ? cccl_type_info{0, 1, CCCL_STORAGE} : input_values_it.value_type;
+  r.order               = sort_order;
+  r.runtime_policy      = policy;
+  r.runtime_policy_size = sizeof(cccl_amd_radix_sort_config_t);
+  const std::string kv  = std::string(width_name(key_bytes)) + "," + width_name(int(vbytes));
+  *name_slot(&r, 0)     = dup_cstr("cccl_amd::block_sort_kernel<" + kv + ">");
+  for (int i = 1; i <= 5; ++i)
+  {
+    *name_slot(&r, i) = dup_cstr("");
+  }
+  *name_slot(&r, 6)  = dup_cstr(std::string("cccl_amd::histogram_kernel<") + width_name(key_bytes) + ">");
+  *name_slot(&r, 7)  = dup_cstr("");
+  *name_slot(&r, 8)  = dup_cstr("cccl_amd::init_kernel");
+  *name_slot(&r, 9)  = dup_cstr("");
+  *name_slot(&r, 10) = dup_cstr("cccl_amd::onesweep_kernel<" + kv + ">");
+  *build             = r;
+  return CUDA_SUCCESS;
+}
+
+bool read_payload(const cccl_device_radix_sort_build_result_t& b, BuildPayload& out)
+{
+  if (b.payload == nullptr || b.payload_size != sizeof(BuildPayload))
+  {
+    return false;
+  }
+  std::memcpy(&out, b.payload, sizeof(out));
+  return std::memcmp(out.magic, kPayloadMagic, 8) == 0 && out.version == 1;
+}
+
+// ---- serialization ----------------------------------------------------------
+constexpr char kBlobMagic[8] = {'C', 'C', 'A', 'M', 'D', 'R', 'S', '1'};
+
+struct Writer
+{
+  std::vector<unsigned char> buf;
+  void bytes(const void* p, size_t n)
+  {
+    const auto* c = static_cast<const unsigned char*>(p);
+    buf.insert(buf.end(), c, c + n);
+  }
+  template <class T>
+  void pod(const T& v)
+  {
+    bytes(&v, sizeof(T));
+  }
+  void blob(const void* p, size_t n)
+  {
+    pod<uint64_t>(n);
+    bytes(p, n);
+  }
+  void cstr(const char* s)
+  {
+    if (s == nullptr)
+    {
+      pod<uint32_t>(0xffffffffu);
+      return;
+    }
+    const uint32_t n = uint32_t(std::strlen(s));
+    pod<uint32_t>(n);
+    bytes(s, n);
+  }
+  void type_info(const cccl_type_info& t)
+  {
+    pod<uint64_t>(t.size);
+    pod<uint64_t>(t.alignment);
+    pod<uint32_t>(uint32_t(t.type));
+  }
+};
+
+struct Reader
+{
+  const unsigned char* p;
+  size_t left;
+  bool ok = true;
+  void bytes(void* dst, size_t n)
+  {
+    if (!ok || n > left)
+    {
+      ok = false;
+      return;
+    }
+    std::memcpy(dst, p, n);
+    p += n;
+    left -= n;
+  }
+  template <class T>
+  T pod()
+  {
+    T v{};
+    bytes(&v, sizeof(T));
+    return v;
+  }
+  char* blob_new(size_t& n)
+  {
+    const uint64_t len = pod<uint64_t>();
+    if (!ok || len > left)
+    {
+      ok = false;
+      return nullptr;
+    }
+    n       = size_t(len);
+    char* d = new char[n > 0 ? n : 1];
+    bytes(d, n);
+    return d;
+  }
+  char* cstr_new()
+  {
+    const uint32_t len = pod<uint32_t>();
+    if (!ok || len == 0xffffffffu)
+    {
+      return nullptr;
+    }
+    if (len > left)
+    {
+      ok = false;
+      return nullptr;
+    }
+    char* d = new char[size_t(len) + 1];
+    bytes(d, len);
+    d[len] = '\0';
+    return d;
+  }
+  cccl_type_info type_info()
+  {
+    cccl_type_info t{};
+    t.size      = size_t(pod<uint64_t>());
+    t.alignment = size_t(pod<uint64_t>());
+    t.type      = cccl_type_enum(pod<uint32_t>());
+    return t;
+  }
+};
+
+} // namespace
+
+extern "C" {
+
+CCCL_C_API CUresult cccl_device_radix_sort_compile(
+  cccl_device_radix_sort_build_result_t* build,
+  cccl_sort_order_t sort_order,
+  cccl_iterator_t input_keys_it,
+  cccl_iterator_t input_values_it,
+  cccl_op_t decomposer,
+  const char* /*decomposer_return_type*/,
+  int cc_major,
+  int cc_minor,
+  const char* /*cub_path*/,
+  const char* /*thrust_path*/,
+  const char* /*libcudacxx_path*/,
+  const char* /*ctk_path*/,
+  cccl_build_config* /*config*/)
+{
+  try
+  {
+    return compile_impl(build, sort_order, input_keys_it, input_values_it, decomposer, cc_major, cc_minor);
+  }
+  catch (const std::exception& exc)
+  {
+    std::printf("\nEXCEPTION in cccl_device_radix_sort_compile(): %s\n", exc.what());
+    std::fflush(stdout);
+    return CUDA_ERROR_UNKNOWN;
+  }
+}
+
+CCCL_C_API CUresult cccl_device_radix_sort_load(cccl_device_radix_sort_build_result_t* build)
+{
+  if (build == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  BuildPayload bp{};
+  if (!read_payload(*build, bp) || build->runtime_policy == nullptr
+      || build->runtime_policy_size != sizeof(cccl_amd_radix_sort_config_t))
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  if (cccl_amd::kernels_for_key_bytes(bp.key_bytes) == nullptr || !value_size_ok(size_t(bp.value_bytes)))
+  {
+    return CUDA_ERROR_INVALID_IMAGE;
+  }
+  build->library = reinterpret_cast<CUlibrary>(const_cast<cccl_amd_library_st_impl*>(&g_library));
+  build->single_tile_kernel            = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_single_tile));
+  build->histogram_kernel              = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_histogram));
+  build->init_bins_and_counters_kernel = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_init));
+  build->onesweep_kernel               = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_onesweep));
+  build->upsweep_kernel = build->alt_upsweep_kernel = build->scan_bins_kernel = nullptr;
+  build->downsweep_kernel = build->alt_downsweep_kernel = nullptr;
+  build->exclusive_sum_kernel = build->init_lookback_kernel = nullptr;
+  return CUDA_SUCCESS;
+}
+
+CCCL_C_API CUresult cccl_device_radix_sort_build_ex(
+  cccl_device_radix_sort_build_result_t* build,
+  cccl_sort_order_t sort_order,
+  cccl_iterator_t input_keys_it,
+  cccl_iterator_t input_values_it,
+  cccl_op_t decomposer,
+  const char* decomposer_return_type,
+  int cc_major,
+  int cc_minor,
+  const char* cub_path,
+  const char* thrust_path,
+  const char* libcudacxx_path,
+  const char* ctk_path,
+  cccl_build_config* config)
+{
+  CUresult r = cccl_device_radix_sort_compile(
+    build,
+    sort_order,
+    input_keys_it,
+    input_values_it,
+    decomposer,
+    decomposer_return_type,
+    cc_major,
+    cc_minor,
+    cub_path,
+    thrust_path,
+    libcudacxx_path,
+    ctk_path,
+    config);
+  if (r != CUDA_SUCCESS)
+  {
+    return r;
+  }
+  r = cccl_device_radix_sort_load(build);
+  if (r != CUDA_SUCCESS)
+  {
+    free_build(build);
+  }
+  return r;
+}
+
+CCCL_C_API CUresult cccl_device_radix_sort_build(
+  cccl_device_radix_sort_build_result_t* build,
+  cccl_sort_order_t sort_order,
+  cccl_iterator_t input_keys_it,
+  cccl_iterator_t input_values_it,
+  cccl_op_t decomposer,
+  const char* decomposer_return_type,
+  int cc_major,
+  int cc_minor,
+  const char* cub_path,
+  const char* thrust_path,
+  const char* libcudacxx_path,
+  const char* ctk_path)
+{
+  return cccl_device_radix_sort_build_ex(
+    build,
+    sort_order,
+    input_keys_it,
+    input_values_it,
+    decomposer,
+    decomposer_return_type,
+    cc_major,
+    cc_minor,
+    cub_path,
+    thrust_path,
+    libcudacxx_path,
+    ctk_path,
+    nullptr);
+}
+
+CCCL_C_API CUresult cccl_device_radix_sort(
+  cccl_device_radix_sort_build_result_t build,
+  void* d_temp_storage,
+  size_t* temp_storage_bytes,
+  cccl_iterator_t d_keys_in,
+  cccl_iterator_t d_keys_out,
+  cccl_iterator_t d_values_in,
+  cccl_iterator_t d_values_out,
+  cccl_op_t /*decomposer*/,
+  uint64_t num_items,
+  int begin_bit,
+  int end_bit,
+  bool is_overwrite_okay,
+  int* selector,
+  CUstream stream)
+{
+  if (d_keys_in.type != CCCL_POINTER || d_values_in.type != CCCL_POINTER || d_keys_out.type != CCCL_POINTER
+      || d_values_out.type != CCCL_POINTER)
+  {
+    std::fflush(stderr);
+    std::printf("\nERROR in cccl_device_radix_sort(): radix sort input must be a pointer\n");
+    std::fflush(stdout);
+    return CUDA_ERROR_UNKNOWN;
+  }
+  if (temp_storage_bytes == nullptr || selector == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  BuildPayload bp{};
+  if (!read_payload(build, bp) || build.runtime_policy == nullptr || build.onesweep_kernel == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  try
+  {
+    SortProblem pr{};
+    pr.key_bytes  = bp.key_bytes;
+    pr.key_kind   = bp.key_kind;
+    pr.val_bytes  = bp.value_bytes;
+    pr.num_items  = num_items;
+    pr.begin_bit  = begin_bit;
+    pr.end_bit    = end_bit;
+    pr.descending = bp.order == CCCL_DESCENDING;
+    pr.overwrite  = is_overwrite_okay;
+    cccl_amd_radix_sort_config_t cfg = *static_cast<const cccl_amd_radix_sort_config_t*>(build.runtime_policy);
+    if (cfg.onesweep_threads == 0)
+    {
+      cfg = cccl_amd::default_config(pr.key_bytes, pr.val_bytes, num_items);
+    }
+    DoubleBufferRef keys{{d_keys_in.state, d_keys_out.state}, 0};
+    DoubleBufferRef vals{{d_values_in.state, d_values_out.state}, 0};
+    if (pr.val_bytes == 0)
+    {
+      vals = DoubleBufferRef{{nullptr, nullptr}, 0};
+    }
+    const hipError_t e = cccl_amd::radix_sort(
+      d_temp_storage, *temp_storage_bytes, keys, vals, pr, cfg, reinterpret_cast<hipStream_t>(stream));
+    *selector = keys.selector;
+    return static_cast<CUresult>(e);
+  }
+  catch (const std::exception& exc)
+  {
+    std::fflush(stderr);
+    std::printf("\nEXCEPTION in cccl_device_radix_sort(): %s\n", exc.what());
+    std::fflush(stdout);
+    return CUDA_ERROR_UNKNOWN;
+  }
+}
+
+CCCL_C_API CUresult cccl_device_radix_sort_link_ltoir(
+  cccl_device_radix_sort_build_result_t* build, const void** /*input_blobs*/, const size_t* /*input_sizes*/, size_t num_inputs)
+{
+  if (build == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  return num_inputs == 0 ? CUDA_SUCCESS : CUDA_ERROR_NOT_SUPPORTED;
+}
+
+CCCL_C_API CUresult
+cccl_device_radix_sort_serialize(const cccl_device_radix_sort_build_result_t* build, void** out_buf, size_t* out_size)
+{
+  if (build == nullptr || out_buf == nullptr || out_size == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  if (build->payload == nullptr || build->payload_size == 0 || build->runtime_policy == nullptr
+      || build->runtime_policy_size == 0)
+  {
+    *out_buf  = nullptr;
+    *out_size = 0;
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  try
+  {
+    Writer w;
+    w.bytes(kBlobMagic, 8);
+    w.pod<int32_t>(build->cc);
+    w.pod<uint32_t>(uint32_t(build->payload_kind));
+    w.type_info(build->key_type);
+    w.type_info(build->value_type);
+    w.pod<uint32_t>(uint32_t(build->order));
+    w.blob(build->payload, build->payload_size);
+    w.blob(build->runtime_policy, build->runtime_policy_size);
+    for (int i = 0; i < kNumNames; ++i)
+    {
+      w.cstr(*name_slot(const_cast<cccl_device_radix_sort_build_result_t*>(build), i));
+    }
+    void* out = std::malloc(w.buf.size());
+    if (out == nullptr)
+    {
+      return CUDA_ERROR_OUT_OF_MEMORY;
+    }
+    std::memcpy(out, w.buf.data(), w.buf.size());
+    *out_buf  = out;
+    *out_size = w.buf.size();
+    return CUDA_SUCCESS;
+  }
+  catch (const std::exception& exc)
+  {
+    std::printf("\nEXCEPTION in cccl_device_radix_sort_serialize(): %s\n", exc.what());
+    std::fflush(stdout);
+    return CUDA_ERROR_UNKNOWN;
+  }
+}
+
+CCCL_C_API CUresult
+cccl_device_radix_sort_deserialize(cccl_device_radix_sort_build_result_t* build, const void* buf, size_t size)
+{
+  if (build == nullptr || buf == nullptr || size == 0)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  cccl_device_radix_sort_build_result_t r{};
+  try
+  {
+    Reader rd{static_cast<const unsigned char*>(buf), size};
+    char magic[8] = {};
+    rd.bytes(magic, 8);
+    if (!rd.ok || std::memcmp(magic, kBlobMagic, 8) != 0)
+    {
+      return CUDA_ERROR_INVALID_VALUE;
+    }
+    r.cc           = rd.pod<int32_t>();
+    r.payload_kind = cccl_payload_kind_t(rd.pod<uint32_t>());
+    r.key_type     = rd.type_info();
+    r.value_type   = rd.type_info();
+    r.order        = cccl_sort_order_t(rd.pod<uint32_t>());
+    if (!rd.ok || (r.order != CCCL_ASCENDING && r.order != CCCL_DESCENDING))
+    {
+      return CUDA_ERROR_INVALID_VALUE;
+    }
+    r.payload = rd.blob_new(r.payload_size);
+    if (!rd.ok || r.payload_size == 0)
+    {
+      free_build(&r);
+      return CUDA_ERROR_INVALID_VALUE;
+    }
+    size_t policy_size = 0;
+    char* policy       = rd.blob_new(policy_size);
+    if (!rd.ok || policy_size != sizeof(cccl_amd_radix_sort_config_t))
+    {
+      delete[] policy;
+      free_build(&r);
+      return CUDA_ERROR_INVALID_VALUE;
+    }
+    r.runtime_policy = std::malloc(policy_size);
+    if (r.runtime_policy == nullptr)
+    {
+      delete[] policy;
+      free_build(&r);
+      return CUDA_ERROR_OUT_OF_MEMORY;
+    }
+    std::memcpy(r.runtime_policy, policy, policy_size);
+    delete[] policy;
+    r.runtime_policy_size = policy_size;
+    for (int i = 0; i < kNumNames; ++i)
+    {
+      *name_slot(&r, i) = rd.cstr_new();
+    }
+    BuildPayload bp{};
+    if (!rd.ok || !read_payload(r, bp))
+    {
+      free_build(&r);
+      return CUDA_ERROR_INVALID_VALUE;
+    }
+    *build = r;
+    return CUDA_SUCCESS;
+  }
+  catch (const std::exception& exc)
+  {
+    free_build(&r);
+    std::printf("\nEXCEPTION in cccl_device_radix_sort_deserialize(): %s\n", exc.what());
+    std::fflush(stdout);
+    return CUDA_ERROR_UNKNOWN;
+  }
+}
+
+CCCL_C_API CUresult cccl_device_radix_sort_cleanup(cccl_device_radix_sort_build_result_t* bld_ptr)
+{
+  if (bld_ptr == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  free_build(bld_ptr);
+  return CUDA_SUCCESS;
+}
+
+CCCL_C_API void cccl_serialization_buffer_free(void* buf)
+{
+  std::free(buf);
+}
+
+// ---- typed entry points -----------------------------------------------------
+
+CCCL_C_API int cccl_amd_radix_sort(
+  void* d_temp_storage,
+  size_t* temp_storage_bytes,
+  void* keys0,
+  void* keys1,
+  void* vals0,
+  void* vals1,
+  int* selector,
+  cccl_type_enum key_type,
+  size_t value_bytes,
+  uint64_t num_items,
+  int begin_bit,
+  int end_bit,
+  int descending,
+  int is_overwrite_okay,
+  const cccl_amd_radix_sort_config_t* cfg,
+  CUstream stream)
+{
+  if (temp_storage_bytes == nullptr || selector == nullptr || (*selector != 0 && *selector != 1))
+  {
+    return hipErrorInvalidValue;
+  }
+  SortProblem pr{};
+  if (!key_traits(key_type, pr.key_bytes, pr.key_kind) || !value_size_ok(value_bytes))
+  {
+    return hipErrorNotSupported;
+  }
+  if (vals0 == nullptr)
+  {
+    value_bytes = 0;
+  }
+  pr.val_bytes  = int(value_bytes);
+  pr.num_items  = num_items;
+  pr.begin_bit  = begin_bit;
+  pr.end_bit    = end_bit;
+  pr.descending = descending != 0;
+  pr.overwrite  = is_overwrite_okay != 0;
+  const cccl_amd_radix_sort_config_t c =
+    cfg != nullptr ? *cfg : cccl_amd::default_config(pr.key_bytes, pr.val_bytes, num_items);
+  DoubleBufferRef keys{{keys0, keys1}, *selector};
+  DoubleBufferRef vals{{vals0, vals1}, *selector};
+  const hipError_t e =
+    cccl_amd::radix_sort(d_temp_storage, *temp_storage_bytes, keys, vals, pr, c, reinterpret_cast<hipStream_t>(stream));
+  *selector = keys.selector;
+  return int(e);
+}
+
+CCCL_C_API int cccl_amd_segmented_radix_sort(
+  void* d_temp_storage,
+  size_t* temp_storage_bytes,
+  void* keys0,
+  void* keys1,
+  void* vals0,
+  void* vals1,
+  int* selector,
+  cccl_type_enum key_type,
+  size_t value_bytes,
+  uint64_t num_items,
+  int64_t num_segments,
+  const void* d_begin_offsets,
+  const void* d_end_offsets,
+  int offset_bytes,
+  int begin_bit,
+  int end_bit,
+  int descending,
+  int is_overwrite_okay,
+  const cccl_amd_segmented_config_t* cfg,
+  CUstream stream)
+{
+  if (temp_storage_bytes == nullptr || selector == nullptr || (*selector != 0 && *selector != 1) || num_segments < 0
+      || (offset_bytes != 4 && offset_bytes != 8))
+  {
+    return hipErrorInvalidValue;
+  }
+  SortProblem pr{};
+  if (!key_traits(key_type, pr.key_bytes, pr.key_kind) || !value_size_ok(value_bytes))
+  {
+    return hipErrorNotSupported;
+  }
+  if (vals0 == nullptr)
+  {
+    value_bytes = 0;
+  }
+  pr.val_bytes  = int(value_bytes);
+  pr.num_items  = num_items;
+  pr.begin_bit  = begin_bit;
+  pr.end_bit    = end_bit;
+  pr.descending = descending != 0;
+  pr.overwrite  = is_overwrite_okay != 0;
+  cccl_amd_segmented_config_t c{};
+  if (cfg != nullptr)
+  {
+    c = *cfg;
+  }
+  else
+  {
+    cccl_amd_segmented_default_config(pr.key_bytes, pr.val_bytes, &c);
+  }
+  DoubleBufferRef keys{{keys0, keys1}, *selector};
+  DoubleBufferRef vals{{vals0, vals1}, *selector};
+  const hipError_t e = cccl_amd::segmented_radix_sort(
+    d_temp_storage,
+    *temp_storage_bytes,
+    keys,
+    vals,
+    pr,
+    num_segments,
+    d_begin_offsets,
+    d_end_offsets,
+    offset_bytes,
+    c,
+    reinterpret_cast<hipStream_t>(stream));
+  *selector = keys.selector;
+  return int(e);
+}
+
+CCCL_C_API void cccl_amd_radix_sort_default_config(
+  int key_bytes, int value_bytes, uint64_t num_items, cccl_amd_radix_sort_config_t* cfg)
+{
+  if (cfg != nullptr)
+  {
+    *cfg = cccl_amd::default_config(key_bytes, value_bytes, num_items);
+  }
+}
+
+CCCL_C_API int cccl_amd_radix_sort_config_valid(const cccl_amd_radix_sort_config_t* cfg, int key_bytes, int value_bytes)
+{
+  return cfg != nullptr && cccl_amd::config_is_valid(*cfg, key_bytes, value_bytes) ? 1 : 0;
+}
+
+CCCL_C_API int cccl_amd_onesweep_shapes(int* threads, int* ipt, int max)
+{
+  static const int t[] = {
+#define CCCL_AMD_X(T, I) T,
+    CCCL_AMD_ONESWEEP_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
+  static const int i[] = {
+#define CCCL_AMD_X(T, I) I,
+    CCCL_AMD_ONESWEEP_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
+  const int n = int(sizeof(t) / sizeof(t[0]));
+  for (int k = 0; k < n && k < max; ++k)
+  {
+    threads[k] = t[k];
+    ipt[k]     = i[k];
+  }
+  return n;
+}
+
+CCCL_C_API int cccl_amd_block_sort_shapes(int* threads, int* ipt, int max)
+{
+  static const int t[] = {
+#define CCCL_AMD_X(T, I) T,
+    CCCL_AMD_BLOCK_SORT_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
+  static const int i[] = {
+#define CCCL_AMD_X(T, I) I,
+    CCCL_AMD_BLOCK_SORT_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
+  const int n = int(sizeof(t) / sizeof(t[0]));
+  for (int k = 0; k < n && k < max; ++k)
+  {
+    threads[k] = t[k];
+    ipt[k]     = i[k];
+  }
+  return n;
+}
+
+} // extern "C"

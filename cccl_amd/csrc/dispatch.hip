@@ -1,0 +1,517 @@
+// Host dispatch of the device-wide radix sort: temp-storage plan, launch sequence,
+// DoubleBuffer bookkeeping, kernel timing hooks.
+#include <algorithm>
+#include <mutex>
+#include <set>
+#include <utility>
+#include <vector>
+
+#include "dispatch.hpp"
+#include "registry.hpp"
+#include "sort_kernels.hpp"
+
+namespace cccl_amd {
+
+const KeyWidthTable* kernels_for_key_bytes(int key_bytes)
+{
+  switch (key_bytes)
+  {
+    case 1:
+      return &kernels_k1();
+    case 2:
+      return &kernels_k2();
+    case 4:
+      return &kernels_k4();
+    case 8:
+      return &kernels_k8();
+    default:
+      return nullptr;
+  }
+}
+
+namespace {
+
+template <class T>
+constexpr T ceil_div(T a, T b)
+{
+  return (a + b - 1) / b;
+}
+
+} // namespace
+
+const KernelEntry* find_entry(const KernelEntry* list, size_t count, int key_bytes, int val_bytes, int threads, int ipt)
+{
+  for (size_t i = 0; i < count; ++i)
+  {
+    const KernelEntry& e = list[i];
+    if (e.key_bytes == key_bytes && e.val_bytes == val_bytes && e.threads == threads && e.ipt == ipt)
+    {
+      return e.fn != nullptr ? &e : nullptr;
+    }
+  }
+  return nullptr;
+}
+
+int device_cu_count()
+{
+  static std::mutex mu;
+  static int cached[64] = {};
+  int dev               = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+  {
+    return 256;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (cached[dev] == 0)
+  {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+    {
+      n = 256;
+    }
+    cached[dev] = n;
+  }
+  return cached[dev];
+}
+
+namespace {
+
+hipError_t ensure_lds_attribute(const void* fn, size_t lds)
+{
+  if (lds <= 64 * 1024)
+  {
+    return hipSuccess;
+  }
+  static std::mutex mu;
+  static std::set<std::pair<int, const void*>> done;
+  int dev = 0;
+  if (hipError_t e = hipGetDevice(&dev); e != hipSuccess)
+  {
+    return e;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (done.count({dev, fn}) != 0)
+  {
+    return hipSuccess;
+  }
+  if (hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(kMaxLdsBytes));
+      e != hipSuccess)
+  {
+    return e;
+  }
+  done.insert({dev, fn});
+  return hipSuccess;
+}
+
+// ---- kernel timing --------------------------------------------------------
+struct ProfileRecord
+{
+  int kind;
+  hipEvent_t start;
+  hipEvent_t stop;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfileRecord> g_prof;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t take_event()
+{
+  if (!g_event_pool.empty())
+  {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void) hipEventCreate(&e);
+  return e;
+}
+
+} // namespace
+
+bool profile_enabled()
+{
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  return g_prof_on;
+}
+
+void profile_record_begin(hipStream_t stream, int kind)
+{
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  if (!g_prof_on)
+  {
+    return;
+  }
+  ProfileRecord r{kind, take_event(), take_event()};
+  (void) hipEventRecord(r.start, stream);
+  g_prof.push_back(r);
+}
+
+void profile_record_end(hipStream_t stream)
+{
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  if (!g_prof_on || g_prof.empty())
+  {
+    return;
+  }
+  (void) hipEventRecord(g_prof.back().stop, stream);
+}
+
+hipError_t launch_kernel(const void* fn, uint64_t grid, int threads, size_t lds, hipStream_t stream, void* params, int kind)
+{
+  if (grid == 0)
+  {
+    return hipSuccess;
+  }
+  if (grid > 0x7fffffffull)
+  {
+    return hipErrorInvalidConfiguration;
+  }
+  if (hipError_t e = ensure_lds_attribute(fn, lds); e != hipSuccess)
+  {
+    return e;
+  }
+  void* args[] = {params};
+  profile_record_begin(stream, kind);
+  hipError_t e = hipLaunchKernel(fn, dim3(uint32_t(grid)), dim3(uint32_t(threads)), args, lds, stream);
+  profile_record_end(stream);
+  if (e != hipSuccess)
+  {
+    return e;
+  }
+  return hipGetLastError();
+}
+
+hipError_t alias_temporaries(void* d_temp, size_t& temp_bytes, void** parts, const size_t* sizes, int count)
+{
+  constexpr size_t kAlign = 256;
+  size_t offsets[16];
+  size_t total = 0;
+  for (int i = 0; i < count; ++i)
+  {
+    offsets[i] = total;
+    total += (sizes[i] + kAlign - 1) & ~(kAlign - 1);
+  }
+  const size_t needed = total + kAlign - 1;
+  if (d_temp == nullptr)
+  {
+    temp_bytes = needed;
+    return hipSuccess;
+  }
+  if (temp_bytes < needed)
+  {
+    return hipErrorInvalidValue;
+  }
+  const uintptr_t base = (reinterpret_cast<uintptr_t>(d_temp) + kAlign - 1) & ~uintptr_t(kAlign - 1);
+  for (int i = 0; i < count; ++i)
+  {
+    parts[i] = reinterpret_cast<void*>(base + offsets[i]);
+  }
+  return hipSuccess;
+}
+
+// Twiddle constants of a key type.
+void twiddle_constants(int key_bytes, int key_kind, bool descending, uint64_t& xor_mask, int& float_mode)
+{
+  const int bits      = key_bytes * 8;
+  const uint64_t high = 1ull << (bits - 1);
+  const uint64_t all  = bits == 64 ? ~0ull : ((1ull << bits) - 1ull);
+  float_mode          = key_kind == kFloatKey ? 1 : 0;
+  xor_mask            = (key_kind == kSignedKey ? high : 0ull) ^ (descending ? all : 0ull);
+}
+
+cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64_t num_items)
+{
+  cccl_amd_radix_sort_config_t c{};
+  c.radix_bits              = kRadixBits;
+  c.histogram_blocks_per_cu = 4;
+  c.single_tile_threads     = 256;
+  c.single_tile_ipt         = 16;
+  const int item_bytes      = key_bytes + val_bytes;
+  if (num_items >= (uint64_t(1) << 22))
+  {
+    c.onesweep_threads = item_bytes > 12 ? 256 : 512;
+    c.onesweep_ipt     = 16;
+  }
+  else
+  {
+    c.onesweep_threads = 256;
+    c.onesweep_ipt     = 8;
+  }
+  return c;
+}
+
+bool config_is_valid(const cccl_amd_radix_sort_config_t& cfg, int key_bytes, int val_bytes)
+{
+  const KeyWidthTable* tab = kernels_for_key_bytes(key_bytes);
+  if (tab == nullptr || cfg.radix_bits != kRadixBits)
+  {
+    return false;
+  }
+  if (find_entry(tab->onesweep, tab->num_onesweep, key_bytes, val_bytes, cfg.onesweep_threads, cfg.onesweep_ipt)
+      == nullptr)
+  {
+    return false;
+  }
+  if (cfg.single_tile_threads > 0
+      && find_entry(
+           tab->block_sort, tab->num_block_sort, key_bytes, val_bytes, cfg.single_tile_threads, cfg.single_tile_ipt)
+           == nullptr)
+  {
+    return false;
+  }
+  return true;
+}
+
+hipError_t radix_sort(
+  void* d_temp_storage,
+  size_t& temp_storage_bytes,
+  DoubleBufferRef& keys,
+  DoubleBufferRef& vals,
+  const SortProblem& pr,
+  const cccl_amd_radix_sort_config_t& cfg,
+  hipStream_t stream)
+{
+  const int kb         = pr.key_bytes;
+  const int vb         = pr.val_bytes;
+  const bool keys_only = vb == 0;
+  const uint64_t n     = pr.num_items;
+  const KeyWidthTable* tab = kernels_for_key_bytes(kb);
+  if (tab == nullptr || (vb != 0 && vb != 1 && vb != 2 && vb != 4 && vb != 8 && vb != 16))
+  {
+    return hipErrorInvalidValue;
+  }
+  if (pr.begin_bit < 0 || pr.end_bit > kb * 8 || pr.begin_bit > pr.end_bit)
+  {
+    return hipErrorInvalidValue;
+  }
+
+  // Nothing to do (dispatch_radix_sort.cuh:1956-1963).
+  if (n == 0 || (pr.begin_bit == pr.end_bit && pr.overwrite))
+  {
+    if (d_temp_storage == nullptr)
+    {
+      temp_storage_bytes = 1;
+    }
+    return hipSuccess;
+  }
+
+  // No bits to sort and no overwrite: copy (dispatch_radix_sort.cuh:1364-1410).
+  if (pr.begin_bit == pr.end_bit)
+  {
+    if (d_temp_storage == nullptr)
+    {
+      temp_storage_bytes = 1;
+      return hipSuccess;
+    }
+    profile_record_begin(stream, 7);
+    hipError_t e = hipMemcpyAsync(keys.alternate(), keys.current(), n * kb, hipMemcpyDeviceToDevice, stream);
+    profile_record_end(stream);
+    if (e != hipSuccess)
+    {
+      return e;
+    }
+    keys.selector ^= 1;
+    if (!keys_only)
+    {
+      e = hipMemcpyAsync(vals.alternate(), vals.current(), n * vb, hipMemcpyDeviceToDevice, stream);
+      if (e != hipSuccess)
+      {
+        return e;
+      }
+    }
+    vals.selector ^= 1;
+    return hipSuccess;
+  }
+
+  uint64_t xor_mask = 0;
+  int float_mode    = 0;
+  twiddle_constants(kb, pr.key_kind, pr.descending, xor_mask, float_mode);
+
+  // One workgroup sorts the whole input (dispatch_radix_sort.cuh:1305-1362).
+  if (cfg.single_tile_threads > 0 && n <= uint64_t(cfg.single_tile_threads) * uint64_t(cfg.single_tile_ipt))
+  {
+    const KernelEntry* e =
+      find_entry(tab->block_sort, tab->num_block_sort, kb, vb, cfg.single_tile_threads, cfg.single_tile_ipt);
+    if (e == nullptr)
+    {
+      return hipErrorInvalidValue;
+    }
+    if (d_temp_storage == nullptr)
+    {
+      temp_storage_bytes = 1;
+      return hipSuccess;
+    }
+    BlockSortParams bp{};
+    bp.keys_in      = keys.current();
+    bp.keys_out     = keys.alternate();
+    bp.vals_in      = keys_only ? nullptr : vals.current();
+    bp.vals_out     = keys_only ? nullptr : vals.alternate();
+    bp.num_items    = n;
+    bp.num_segments = 1;
+    bp.xor_mask     = xor_mask;
+    bp.float_mode   = float_mode;
+    bp.begin_bit    = pr.begin_bit;
+    bp.end_bit      = pr.end_bit;
+    if (hipError_t err = launch_kernel(e->fn, 1, e->threads, e->lds_bytes, stream, &bp, 3); err != hipSuccess)
+    {
+      return err;
+    }
+    keys.selector ^= 1;
+    vals.selector ^= 1;
+    return hipSuccess;
+  }
+
+  // Onesweep (dispatch_radix_sort.cuh:1698-1948).
+  const KernelEntry* os =
+    find_entry(tab->onesweep, tab->num_onesweep, kb, vb, cfg.onesweep_threads, cfg.onesweep_ipt);
+  if (os == nullptr || cfg.radix_bits != kRadixBits)
+  {
+    return hipErrorInvalidValue;
+  }
+  const uint64_t tile         = uint64_t(os->threads) * uint64_t(os->ipt);
+  const uint64_t portion_size = (kMaxPortionItems / tile) * tile;
+  const int passes            = ceil_div(pr.end_bit - pr.begin_bit, kRadixBits);
+  const uint64_t portions     = ceil_div(n, portion_size);
+  const uint64_t max_tiles    = ceil_div(std::min(n, portion_size), tile);
+  const bool need_tmp         = !pr.overwrite && passes > 1;
+
+  size_t sizes[7] = {
+    size_t(passes) * kRadix * sizeof(uint64_t), // digit counts of every pass
+    size_t(portions) * passes * kRadix * sizeof(uint64_t), // digit starts of every portion (portions > 0)
+    size_t(max_tiles) * kRadix * sizeof(uint32_t), // look-back buffer A
+    size_t(max_tiles) * kRadix * sizeof(uint32_t), // look-back buffer B
+    need_tmp ? size_t(n) * kb : 0, // extra key buffer
+    need_tmp && !keys_only ? size_t(n) * vb : 0, // extra value buffer
+    size_t(portions) * passes * sizeof(uint32_t), // tile counters
+  };
+  void* parts[7] = {};
+  if (hipError_t e = alias_temporaries(d_temp_storage, temp_storage_bytes, parts, sizes, 7); e != hipSuccess)
+  {
+    return e;
+  }
+  if (d_temp_storage == nullptr)
+  {
+    return hipSuccess;
+  }
+  uint64_t* d_bins     = static_cast<uint64_t*>(parts[0]);
+  uint64_t* d_starts   = static_cast<uint64_t*>(parts[1]);
+  uint32_t* d_lb[2]    = {static_cast<uint32_t*>(parts[2]), static_cast<uint32_t*>(parts[3])};
+  void* d_keys_tmp2    = parts[4];
+  void* d_vals_tmp2    = parts[5];
+  uint32_t* d_counters = static_cast<uint32_t*>(parts[6]);
+
+  {
+    InitParams ip{d_bins, uint64_t(passes) * kRadix, d_counters, portions * passes, d_lb[0], max_tiles * kRadix};
+    const uint64_t most = std::max<uint64_t>(ip.num_lookback, ip.num_bins);
+    const uint64_t grid = std::min<uint64_t>(ceil_div<uint64_t>(most, 256), 2048);
+    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&init_kernel), grid, 256, 0, stream, &ip, 0);
+        e != hipSuccess)
+    {
+      return e;
+    }
+  }
+  {
+    HistogramParams hp{keys.current(), n, d_bins, xor_mask, float_mode, pr.begin_bit, pr.end_bit, passes};
+    const uint64_t per_block = uint64_t(kHistThreads) * kHistUnroll;
+    const uint64_t cap       = uint64_t(device_cu_count()) * uint64_t(std::max(1, cfg.histogram_blocks_per_cu));
+    const uint64_t grid      = std::max<uint64_t>(1, std::min<uint64_t>(cap, ceil_div(n, per_block)));
+    if (hipError_t e = launch_kernel(tab->histogram, grid, kHistThreads, 0, stream, &hp, 1); e != hipSuccess)
+    {
+      return e;
+    }
+  }
+
+  void* d_keys_tmp = keys.alternate();
+  void* d_vals_tmp = vals.alternate();
+  if (!pr.overwrite && passes % 2 == 0)
+  {
+    keys.buf[1] = d_keys_tmp2;
+    vals.buf[1] = d_vals_tmp2;
+  }
+
+  int launch_index = 0;
+  for (int bit = pr.begin_bit, pass = 0; bit < pr.end_bit; bit += kRadixBits, ++pass)
+  {
+    const int num_bits = std::min(pr.end_bit - bit, kRadixBits);
+    for (uint64_t portion = 0; portion < portions; ++portion)
+    {
+      const uint64_t first = portion * portion_size;
+      const uint64_t items = std::min(n - first, portion_size);
+      const uint64_t tiles = ceil_div(items, tile);
+      OnesweepParams op{};
+      op.keys_in         = static_cast<const char*>(keys.current()) + first * kb;
+      op.keys_out        = keys.alternate();
+      op.vals_in         = keys_only ? nullptr : static_cast<const char*>(vals.current()) + first * vb;
+      op.vals_out        = keys_only ? nullptr : vals.alternate();
+      op.lookback        = d_lb[launch_index & 1];
+      op.lookback_next   = d_lb[(launch_index & 1) ^ 1];
+      op.tile_counter    = d_counters + portion * passes + pass;
+      op.bins            = portion == 0 ? d_bins + size_t(pass) * kRadix : d_starts + (portion * passes + pass) * kRadix;
+      op.bins_next       = portion + 1 < portions ? d_starts + ((portion + 1) * passes + pass) * kRadix : nullptr;
+      op.num_items       = items;
+      op.xor_mask        = xor_mask;
+      op.num_tiles       = uint32_t(tiles);
+      op.zero_tiles      = uint32_t(max_tiles);
+      op.float_mode      = float_mode;
+      op.shift           = bit;
+      op.num_bits        = num_bits;
+      op.bins_are_starts = portion == 0 ? 0 : 1;
+      if (hipError_t e = launch_kernel(os->fn, tiles, os->threads, os->lds_bytes, stream, &op, 2); e != hipSuccess)
+      {
+        return e;
+      }
+      ++launch_index;
+    }
+    if (!pr.overwrite && pass == 0)
+    {
+      keys = passes % 2 == 0 ? DoubleBufferRef{{d_keys_tmp, d_keys_tmp2}, 0} : DoubleBufferRef{{d_keys_tmp2, d_keys_tmp}, 0};
+      vals = passes % 2 == 0 ? DoubleBufferRef{{d_vals_tmp, d_vals_tmp2}, 0} : DoubleBufferRef{{d_vals_tmp2, d_vals_tmp}, 0};
+    }
+    keys.selector ^= 1;
+    vals.selector ^= 1;
+  }
+  return hipSuccess;
+}
+
+} // namespace cccl_amd
+
+// ---- C entry points of the profiling hooks ----------------------------------
+extern "C" {
+
+CCCL_C_API void cccl_amd_profile_enable(int on)
+{
+  std::lock_guard<std::mutex> lock(cccl_amd::g_prof_mu);
+  cccl_amd::g_prof_on = on != 0;
+}
+
+CCCL_C_API int cccl_amd_profile_read(int* kinds, float* ms, int max)
+{
+  std::lock_guard<std::mutex> lock(cccl_amd::g_prof_mu);
+  const int count = int(cccl_amd::g_prof.size());
+  for (int i = 0; i < count; ++i)
+  {
+    cccl_amd::ProfileRecord& r = cccl_amd::g_prof[i];
+    (void) hipEventSynchronize(r.stop);
+    if (i < max)
+    {
+      float t = 0.f;
+      (void) hipEventElapsedTime(&t, r.start, r.stop);
+      if (kinds != nullptr)
+      {
+        kinds[i] = r.kind;
+      }
+      if (ms != nullptr)
+      {
+        ms[i] = t;
+      }
+    }
+    cccl_amd::g_event_pool.push_back(r.start);
+    cccl_amd::g_event_pool.push_back(r.stop);
+  }
+  cccl_amd::g_prof.clear();
+  return count;
+}
+
+} // extern "C"

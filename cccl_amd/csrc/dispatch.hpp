@@ -1,0 +1,97 @@
+// Host dispatch of the radix sort (internal C++ interface behind the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/cccl_amd/radix_sort_amd.h"
+
+namespace cccl_amd {
+
+enum KeyKind : int
+{
+  kUnsignedKey = 0,
+  kSignedKey   = 1,
+  kFloatKey    = 2,
+};
+
+// Mirrors cub::DoubleBuffer (cub/cub/util_type.cuh:750-778): buf[selector] is current.
+struct DoubleBufferRef
+{
+  void* buf[2];
+  int selector;
+  void* current() const
+  {
+    return buf[selector];
+  }
+  void* alternate() const
+  {
+    return buf[selector ^ 1];
+  }
+};
+
+struct SortProblem
+{
+  int key_bytes;
+  int key_kind;
+  int val_bytes; // 0 = keys only
+  uint64_t num_items;
+  int begin_bit;
+  int end_bit;
+  bool descending;
+  bool overwrite; // is_overwrite_okay
+};
+
+// Device-wide sort.  Follows cub/cub/device/dispatch/dispatch_radix_sort.cuh:1951-2005
+// (edge paths) and :1698-1948 (onesweep temp layout, pass/portion loop, DoubleBuffer
+// juggling).  keys/vals are updated in place like CUB's DoubleBuffer arguments.
+hipError_t radix_sort(
+  void* d_temp_storage,
+  size_t& temp_storage_bytes,
+  DoubleBufferRef& keys,
+  DoubleBufferRef& vals,
+  const SortProblem& prob,
+  const cccl_amd_radix_sort_config_t& cfg,
+  hipStream_t stream);
+
+// Segmented sort, one independent stable sort per [begin[i], end[i]).
+hipError_t segmented_radix_sort(
+  void* d_temp_storage,
+  size_t& temp_storage_bytes,
+  DoubleBufferRef& keys,
+  DoubleBufferRef& vals,
+  const SortProblem& prob,
+  int64_t num_segments,
+  const void* d_begin_offsets,
+  const void* d_end_offsets,
+  int offset_bytes,
+  const cccl_amd_segmented_config_t& cfg,
+  hipStream_t stream);
+
+cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64_t num_items);
+
+bool config_is_valid(const cccl_amd_radix_sort_config_t& cfg, int key_bytes, int val_bytes);
+
+// Temp bytes for one aligned carve-up (each part rounded to 256 B, plus 255 B slack);
+// mirrors cub/cub/util_temporary_storage.cuh:49-99.
+hipError_t alias_temporaries(void* d_temp, size_t& temp_bytes, void** parts, const size_t* sizes, int count);
+
+struct KernelEntry;
+
+// Compiled kernel for (key bytes, value bytes, threads, ipt), or null.
+const KernelEntry* find_entry(const KernelEntry* list, size_t count, int key_bytes, int val_bytes, int threads, int ipt);
+
+int device_cu_count();
+
+// Launches fn<<<grid, threads, lds, stream>>>(*params); kind tags the timing record.
+hipError_t launch_kernel(const void* fn, uint64_t grid, int threads, size_t lds, hipStream_t stream, void* params, int kind);
+
+void twiddle_constants(int key_bytes, int key_kind, bool descending, uint64_t& xor_mask, int& float_mode);
+
+// Optional per-launch timing used by benchmarks.
+void profile_record_begin(hipStream_t stream, int kind);
+void profile_record_end(hipStream_t stream);
+bool profile_enabled();
+
+} // namespace cccl_amd

@@ -1,0 +1,102 @@
+// Instantiates every kernel of one key width; included once per inst_k*.hip.
+#pragma once
+
+#include <type_traits>
+
+#include "registry.hpp"
+#include "sort_kernels.hpp"
+
+namespace cccl_amd {
+namespace inst {
+
+template <class V>
+constexpr int value_bytes()
+{
+  if constexpr (std::is_same<V, NoValue>::value)
+  {
+    return 0;
+  }
+  else
+  {
+    return int(sizeof(V));
+  }
+}
+
+template <class UK, class V, int T, int I>
+KernelEntry onesweep_entry()
+{
+  using L = OnesweepLayout<UK, V, T, I>;
+  if constexpr (L::kBytes <= kMaxLdsBytes)
+  {
+    return {
+      int(sizeof(UK)), value_bytes<V>(), T, I, reinterpret_cast<const void*>(&onesweep_kernel<UK, V, T, I, false>), L::kBytes};
+  }
+  else
+  {
+    return {int(sizeof(UK)), value_bytes<V>(), T, I, nullptr, L::kBytes};
+  }
+}
+
+template <class UK, class V, int T, int I>
+KernelEntry seg_onesweep_entry()
+{
+  using L = OnesweepLayout<UK, V, T, I>;
+  if constexpr (L::kBytes <= kMaxLdsBytes)
+  {
+    return {
+      int(sizeof(UK)), value_bytes<V>(), T, I, reinterpret_cast<const void*>(&onesweep_kernel<UK, V, T, I, true>), L::kBytes};
+  }
+  else
+  {
+    return {int(sizeof(UK)), value_bytes<V>(), T, I, nullptr, L::kBytes};
+  }
+}
+
+template <class UK, class V, int T, int I>
+KernelEntry block_sort_entry()
+{
+  using L = BlockSortLayout<UK, V, T, I>;
+  if constexpr (L::kBytes <= kMaxLdsBytes)
+  {
+    return {int(sizeof(UK)), value_bytes<V>(), T, I, reinterpret_cast<const void*>(&block_sort_kernel<UK, V, T, I>), L::kBytes};
+  }
+  else
+  {
+    return {int(sizeof(UK)), value_bytes<V>(), T, I, nullptr, L::kBytes};
+  }
+}
+
+#define CCCL_AMD_ALL_VALUES(MAKE, UK, T, I)                                                          \
+  MAKE<UK, NoValue, T, I>(), MAKE<UK, uint8_t, T, I>(), MAKE<UK, uint16_t, T, I>(),                  \
+    MAKE<UK, uint32_t, T, I>(), MAKE<UK, uint64_t, T, I>(), MAKE<UK, Value16, T, I>(),
+
+template <class UK>
+KeyWidthTable make_table()
+{
+  static const KernelEntry onesweep[] = {
+#define CCCL_AMD_X(T, I) CCCL_AMD_ALL_VALUES(onesweep_entry, UK, T, I)
+    CCCL_AMD_ONESWEEP_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
+  static const KernelEntry block_sort[] = {
+#define CCCL_AMD_X(T, I) CCCL_AMD_ALL_VALUES(block_sort_entry, UK, T, I)
+    CCCL_AMD_BLOCK_SORT_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
+  static const KernelEntry seg_onesweep[] = {
+#define CCCL_AMD_X(T, I) CCCL_AMD_ALL_VALUES(seg_onesweep_entry, UK, T, I)
+    CCCL_AMD_SEG_ONESWEEP_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
+  return {reinterpret_cast<const void*>(&histogram_kernel<UK>),
+          onesweep,
+          sizeof(onesweep) / sizeof(onesweep[0]),
+          block_sort,
+          sizeof(block_sort) / sizeof(block_sort[0]),
+          reinterpret_cast<const void*>(&seg_histogram_kernel<UK>),
+          seg_onesweep,
+          sizeof(seg_onesweep) / sizeof(seg_onesweep[0])};
+}
+
+} // namespace inst
+} // namespace cccl_amd

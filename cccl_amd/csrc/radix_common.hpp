@@ -1,0 +1,204 @@
+// Device and host helpers shared by the gfx950 radix-sort kernels.
+//
+// Key transform ("twiddle"): every supported key type maps bijectively onto an
+// unsigned integer of the same width whose unsigned order is the key order.
+// Follows cub/cub/util_type.cuh:846-965 (unsigned: identity; signed: flip the sign
+// bit; floating point: flip all bits of negatives, the sign bit of non-negatives),
+// cub/cub/block/radix_rank_sort_operations.cuh:534-575 (descending: bitwise NOT after
+// the transform) and :43-84 (-0.0 is read as +0.0 when extracting digits, so the two
+// zeros share every digit and keep their input order; stored keys are never changed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cccl_amd {
+
+constexpr int kRadixBits = 8;
+constexpr int kRadix     = 1 << kRadixBits;
+
+// Look-back status word: bit 31 = inclusive prefix over all earlier tiles is final,
+// bit 30 = this tile's own count only, low 30 bits = the count.  Zero = not ready.
+// Same encoding as cub/cub/agent/agent_radix_sort_onesweep.cuh:167-170.
+constexpr uint32_t kFlagGlobal  = 1u << 31;
+constexpr uint32_t kFlagPartial = 1u << 30;
+constexpr uint32_t kCountMask   = kFlagPartial - 1u;
+
+// Largest number of items one onesweep launch may cover: counts in a status word are
+// 30 bits wide, so a launch (a "portion") holds at most 2^30 - 1 items.
+constexpr uint64_t kMaxPortionItems = (1ull << 30) - 1ull;
+
+struct NoValue
+{};
+
+struct alignas(16) Value16
+{
+  uint64_t lo, hi;
+};
+
+// Per-call key transform.  float_mode selects the floating-point rule; xor_mask holds
+// the sign flip for signed keys and the all-ones mask for descending order.
+template <class UK>
+struct Twiddle
+{
+  UK xor_mask;
+  int float_mode;
+
+  static constexpr int kBits = int(sizeof(UK) * 8);
+  static constexpr UK kHigh  = UK(UK(1) << (kBits - 1));
+
+  __host__ __device__ __forceinline__ UK operator()(UK k) const
+  {
+    if (float_mode)
+    {
+      if (k == kHigh)
+      {
+        k = UK(0); // -0.0 reads as +0.0
+      }
+      const UK neg = (k & kHigh) ? UK(~UK(0)) : kHigh;
+      k            = UK(k ^ neg);
+    }
+    return UK(k ^ xor_mask);
+  }
+};
+
+template <class UK>
+__host__ __device__ __forceinline__ uint32_t extract_digit(UK t, int shift, uint32_t mask)
+{
+  return uint32_t(t >> shift) & mask;
+}
+
+// ---------------------------------------------------------------------------
+// wave64 primitives
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t lane_id()
+{
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of m below this lane.
+__device__ __forceinline__ uint32_t mask_prefix_count(uint64_t m)
+{
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
+// Set of lanes of the wave whose 8-bit digit equals this lane's: one ballot per bit;
+// a lane keeps the lanes that agree with it on that bit.  Lanes whose bit is clear
+// keep the complement of the ballot, lanes whose bit is set keep the ballot itself,
+// written as m ^ (bit - 1) so the select is an xor with a per-lane 0 / ~0 value.
+template <int BITS>
+__device__ __forceinline__ uint64_t wave_match_digit(uint32_t d)
+{
+  uint64_t peers = __builtin_amdgcn_read_exec();
+#pragma unroll
+  for (int b = 0; b < BITS; ++b)
+  {
+    const uint32_t bit    = (d >> b) & 1u;
+    const uint64_t m      = __builtin_amdgcn_ballot_w64(bit != 0u);
+    const uint64_t select = uint64_t(int64_t(int32_t(bit) - 1)); // bit ? 0 : ~0
+    peers &= m ^ select;
+  }
+  return peers;
+}
+
+// Inclusive scan across the 64 lanes of a wave.
+template <class T>
+__device__ __forceinline__ T wave_inclusive_scan(T v)
+{
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1)
+  {
+    const T t = __shfl_up(v, o, 64);
+    if (lane >= uint32_t(o))
+    {
+      v += t;
+    }
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread over the first kRadix threads of a block
+// (threads >= kRadix contribute nothing).  s_wave must hold kRadix/64 entries.  Must
+// be reached by every thread of the block.
+template <class T>
+__device__ __forceinline__ T radix_exclusive_scan(T v, T* s_wave, T& total)
+{
+  const uint32_t tid  = threadIdx.x;
+  const uint32_t wave = tid >> 6;
+  T inc               = T(0);
+  if (tid < uint32_t(kRadix))
+  {
+    inc = wave_inclusive_scan<T>(v);
+    if ((tid & 63u) == 63u)
+    {
+      s_wave[wave] = inc;
+    }
+  }
+  __syncthreads();
+  T before = T(0);
+  T all    = T(0);
+#pragma unroll
+  for (int w = 0; w < kRadix / 64; ++w)
+  {
+    const T s = s_wave[w];
+    if (uint32_t(w) < wave)
+    {
+      before += s;
+    }
+    all += s;
+  }
+  total = all;
+  __syncthreads();
+  return before + inc - v;
+}
+
+// Relaxed device-scope accesses for the look-back status words and counters.  They
+// go to the coherence point shared by all XCDs (no per-XCD L2 copy).
+__device__ __forceinline__ uint32_t load_status(const uint32_t* p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_status(uint32_t* p, uint32_t v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave-private LDS counters are read and written by different lanes of the same wave
+// in program order; wavefront-scope relaxed atomics keep the compiler from reordering
+// or caching them and compile to plain ds_read / ds_write.
+__device__ __forceinline__ uint32_t lds_load_wave(const uint32_t* p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+__device__ __forceinline__ void lds_store_wave(uint32_t* p, uint32_t v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// Stable rank of IPT digits per lane within one wave.  Item i of lane l stands at
+// position i*64 + l of the wave's strip, so items are visited in position order:
+// slot by slot, and within a slot by lane.  wc[d] counts the digit-d items seen so
+// far; the highest peer lane writes the new count.
+template <int IPT>
+__device__ __forceinline__ void wave_rank(const uint32_t (&dig)[IPT], uint32_t (&rank)[IPT], uint32_t* wc)
+{
+#pragma unroll
+  for (int i = 0; i < IPT; ++i)
+  {
+    const uint64_t peers = wave_match_digit<kRadixBits>(dig[i]);
+    const uint32_t below = mask_prefix_count(peers);
+    const uint32_t cnt   = uint32_t(__builtin_popcountll(peers));
+    const uint32_t old   = lds_load_wave(wc + dig[i]);
+    rank[i]              = old + below;
+    if (below + 1u == cnt)
+    {
+      lds_store_wave(wc + dig[i], old + cnt);
+    }
+  }
+}
+
+} // namespace cccl_amd

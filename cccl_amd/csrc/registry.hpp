@@ -1,0 +1,67 @@
+// Table of precompiled gfx950 kernel instantiations.
+//
+// Each key width has its own translation unit (inst_k1.hip ... inst_k8.hip) that
+// instantiates the histogram, onesweep and block-sort kernels for every value width
+// and every launch shape listed below, and returns them as a flat table.  The host
+// dispatch picks an entry by (key bytes, value bytes, threads, items per thread).
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+namespace cccl_amd {
+
+// Largest dynamic LDS a gfx950 workgroup may use.
+constexpr size_t kMaxLdsBytes = 160 * 1024;
+
+struct KernelEntry
+{
+  int key_bytes;
+  int val_bytes; // 0 = keys only
+  int threads;
+  int ipt;
+  const void* fn; // null when the shape does not fit in LDS
+  size_t lds_bytes;
+};
+
+struct KeyWidthTable
+{
+  const void* histogram;
+  const KernelEntry* onesweep;
+  size_t num_onesweep;
+  const KernelEntry* block_sort;
+  size_t num_block_sort;
+  const void* seg_histogram;
+  const KernelEntry* seg_onesweep;
+  size_t num_seg_onesweep;
+};
+
+const KeyWidthTable& kernels_k1();
+const KeyWidthTable& kernels_k2();
+const KeyWidthTable& kernels_k4();
+const KeyWidthTable& kernels_k8();
+
+const KeyWidthTable* kernels_for_key_bytes(int key_bytes);
+
+// Launch shapes compiled for the onesweep pass (threads, items per thread).
+#define CCCL_AMD_ONESWEEP_SHAPES(X) \
+  X(256, 8)                         \
+  X(256, 16)                        \
+  X(512, 8)                         \
+  X(512, 12)                        \
+  X(512, 16)
+
+// Launch shapes compiled for the batched onesweep of large segments.
+#define CCCL_AMD_SEG_ONESWEEP_SHAPES(X) \
+  X(256, 16)                            \
+  X(512, 8)                             \
+  X(512, 16)
+
+// Launch shapes compiled for the single-workgroup sort.
+#define CCCL_AMD_BLOCK_SORT_SHAPES(X) \
+  X(256, 4)                           \
+  X(256, 8)                           \
+  X(256, 16)                          \
+  X(512, 16)
+
+} // namespace cccl_amd

@@ -1,0 +1,337 @@
+// Host dispatch of the batched segmented radix sort.
+//
+// Launch sequence (stream-ordered, no host synchronisation, no memset nodes, so a
+// caller can capture it in a hipGraph and replay it):
+//   seg_init      zero the class counters, tile counters and the first look-back buffer
+//   seg_classify  bin every segment by size into the one-workgroup classes or the large
+//                 class; large segments get a contiguous range of onesweep tiles
+//   seg_prep      per large segment: clear its digit counts, map its tiles to it
+//   seg_histogram per large tile: digit counts of every pass into its segment's counts
+//   block_sort    one launch per one-workgroup class, one workgroup per segment
+//   onesweep      one batched launch per digit pass over every large segment's tiles
+// Pass count and final-buffer rules follow
+// cub/cub/device/dispatch/dispatch_segmented_radix_sort.cuh:678-893 with 8-bit digits:
+// with is_overwrite_okay the result ends in buf[(selector + passes) & 1], otherwise in
+// the alternate buffer (selector + 1).
+#include <algorithm>
+
+#include "dispatch.hpp"
+#include "registry.hpp"
+#include "sort_kernels.hpp"
+
+namespace cccl_amd {
+
+namespace {
+
+template <class T>
+constexpr T ceil_div(T a, T b)
+{
+  return (a + b - 1) / b;
+}
+
+} // namespace
+
+hipError_t segmented_radix_sort(
+  void* d_temp_storage,
+  size_t& temp_storage_bytes,
+  DoubleBufferRef& keys,
+  DoubleBufferRef& vals,
+  const SortProblem& pr,
+  int64_t num_segments,
+  const void* d_begin_offsets,
+  const void* d_end_offsets,
+  int offset_bytes,
+  const cccl_amd_segmented_config_t& cfg,
+  hipStream_t stream)
+{
+  const int kb             = pr.key_bytes;
+  const int vb             = pr.val_bytes;
+  const bool keys_only     = vb == 0;
+  const uint64_t n         = pr.num_items;
+  const KeyWidthTable* tab = kernels_for_key_bytes(kb);
+  if (tab == nullptr || (vb != 0 && vb != 1 && vb != 2 && vb != 4 && vb != 8 && vb != 16))
+  {
+    return hipErrorInvalidValue;
+  }
+  if (pr.begin_bit < 0 || pr.end_bit > kb * 8 || pr.begin_bit > pr.end_bit || num_segments < 0
+      || (offset_bytes != 4 && offset_bytes != 8))
+  {
+    return hipErrorInvalidValue;
+  }
+  if (cfg.num_block_classes < 0 || cfg.num_block_classes > kMaxBlockClasses)
+  {
+    return hipErrorInvalidValue;
+  }
+
+  // dispatch_segmented_radix_sort.cuh:930-937
+  if (n == 0 || num_segments == 0 || (pr.begin_bit == pr.end_bit && pr.overwrite))
+  {
+    if (d_temp_storage == nullptr)
+    {
+      temp_storage_bytes = 1;
+    }
+    return hipSuccess;
+  }
+  if (num_segments > int64_t(0x7fffffff))
+  {
+    return hipErrorInvalidValue;
+  }
+
+  const KernelEntry* cls[kMaxBlockClasses] = {};
+  uint32_t class_max[kMaxBlockClasses]     = {};
+  for (int c = 0; c < cfg.num_block_classes; ++c)
+  {
+    cls[c] = find_entry(tab->block_sort, tab->num_block_sort, kb, vb, cfg.block_threads[c], cfg.block_ipt[c]);
+    if (cls[c] == nullptr)
+    {
+      return hipErrorInvalidValue;
+    }
+    class_max[c] = uint32_t(cls[c]->threads * cls[c]->ipt);
+    if (c > 0 && class_max[c] <= class_max[c - 1])
+    {
+      return hipErrorInvalidValue;
+    }
+  }
+  const KernelEntry* large =
+    find_entry(tab->seg_onesweep, tab->num_seg_onesweep, kb, vb, cfg.large_threads, cfg.large_ipt);
+  if (large == nullptr)
+  {
+    return hipErrorInvalidValue;
+  }
+
+  const int passes           = std::max(ceil_div(pr.end_bit - pr.begin_bit, kRadixBits), 1);
+  const uint64_t large_tile  = uint64_t(large->threads) * large->ipt;
+  const uint64_t block_max   = cfg.num_block_classes > 0 ? class_max[cfg.num_block_classes - 1] : 0;
+  // Large segments have more than block_max items each.
+  const uint64_t large_bound = std::min<uint64_t>(uint64_t(num_segments), n / (block_max + 1));
+  // sum over large segments of ceil(size / tile) <= n / tile + their count
+  const uint64_t tile_bound  = large_bound > 0 ? ceil_div(n, large_tile) + large_bound : 0;
+  if (tile_bound > 0x7fffffffull)
+  {
+    return hipErrorInvalidValue;
+  }
+  const int num_counters = kSegCounters + passes;
+
+  size_t sizes[8 + kMaxBlockClasses] = {
+    size_t(num_counters) * sizeof(uint32_t),
+    size_t(large_bound) * 2 * sizeof(int64_t), // large segment ranges
+    size_t(large_bound) * sizeof(uint32_t), // first tile of each large segment
+    size_t(tile_bound) * sizeof(uint32_t), // segment slot of each tile
+    size_t(large_bound) * passes * kRadix * sizeof(uint32_t), // digit counts of each large segment
+    size_t(tile_bound) * kRadix * sizeof(uint32_t), // look-back A
+    size_t(tile_bound) * kRadix * sizeof(uint32_t), // look-back B
+    pr.overwrite ? 0 : size_t(n) * (kb + vb), // extra key + value buffer
+  };
+  for (int c = 0; c < cfg.num_block_classes; ++c)
+  {
+    sizes[8 + c] = size_t(num_segments) * 2 * sizeof(int64_t);
+  }
+  void* parts[8 + kMaxBlockClasses] = {};
+  if (hipError_t e = alias_temporaries(d_temp_storage, temp_storage_bytes, parts, sizes, 8 + kMaxBlockClasses);
+      e != hipSuccess)
+  {
+    return e;
+  }
+  if (d_temp_storage == nullptr)
+  {
+    return hipSuccess;
+  }
+  uint32_t* d_counters    = static_cast<uint32_t*>(parts[0]);
+  int64_t* d_large_range  = static_cast<int64_t*>(parts[1]);
+  uint32_t* d_large_first = static_cast<uint32_t*>(parts[2]);
+  uint32_t* d_tile_slot   = static_cast<uint32_t*>(parts[3]);
+  uint32_t* d_seg_bins    = static_cast<uint32_t*>(parts[4]);
+  uint32_t* d_lb[2]       = {static_cast<uint32_t*>(parts[5]), static_cast<uint32_t*>(parts[6])};
+  char* d_tmp_keys        = static_cast<char*>(parts[7]);
+  char* d_tmp_vals        = d_tmp_keys + size_t(n) * kb;
+
+  uint64_t xor_mask = 0;
+  int float_mode    = 0;
+  twiddle_constants(kb, pr.key_kind, pr.descending, xor_mask, float_mode);
+
+  // Buffers: input I, final F, and the ping-pong sequence of the large class.
+  void* key_in  = keys.current();
+  void* val_in  = keys_only ? nullptr : vals.current();
+  void* key_seq[2];
+  void* val_seq[2];
+  if (pr.overwrite)
+  {
+    key_seq[0] = keys.alternate();
+    key_seq[1] = keys.current();
+    val_seq[0] = keys_only ? nullptr : vals.alternate();
+    val_seq[1] = keys_only ? nullptr : vals.current();
+  }
+  else
+  {
+    void* kf = keys.alternate();
+    void* vf = keys_only ? nullptr : vals.alternate();
+    void* vt = keys_only ? nullptr : static_cast<void*>(d_tmp_vals);
+    key_seq[0] = (passes & 1) ? kf : static_cast<void*>(d_tmp_keys);
+    key_seq[1] = (passes & 1) ? static_cast<void*>(d_tmp_keys) : kf;
+    val_seq[0] = (passes & 1) ? vf : vt;
+    val_seq[1] = (passes & 1) ? vt : vf;
+  }
+  void* key_final = key_seq[(passes - 1) & 1];
+  void* val_final = val_seq[(passes - 1) & 1];
+
+  {
+    SegInitParams ip{d_counters, uint32_t(num_counters), d_lb[0], tile_bound * kRadix};
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(ceil_div<uint64_t>(ip.num_lookback, 256), 2048));
+    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&seg_init_kernel), grid, 256, 0, stream, &ip, 4);
+        e != hipSuccess)
+    {
+      return e;
+    }
+  }
+  {
+    SegClassifyParams cp{};
+    cp.begin_offsets = d_begin_offsets;
+    cp.end_offsets   = d_end_offsets;
+    cp.num_segments  = num_segments;
+    cp.offset_bytes  = offset_bytes;
+    cp.num_classes   = cfg.num_block_classes;
+    for (int c = 0; c < cfg.num_block_classes; ++c)
+    {
+      cp.class_max[c]   = class_max[c];
+      cp.class_range[c] = static_cast<int64_t*>(parts[8 + c]);
+    }
+    cp.large_tile       = uint32_t(large_tile);
+    cp.counters         = d_counters;
+    cp.large_range      = d_large_range;
+    cp.large_first_tile = d_large_first;
+    const uint64_t grid = ceil_div<uint64_t>(uint64_t(num_segments), 256);
+    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&seg_classify_kernel), grid, 256, 0, stream, &cp, 4);
+        e != hipSuccess)
+    {
+      return e;
+    }
+  }
+  if (large_bound > 0)
+  {
+    SegPrepParams pp{d_counters, d_large_range, d_large_first, d_tile_slot, d_seg_bins, uint32_t(large_tile), passes};
+    if (hipError_t e =
+          launch_kernel(reinterpret_cast<const void*>(&seg_prep_kernel), large_bound, 256, 0, stream, &pp, 4);
+        e != hipSuccess)
+    {
+      return e;
+    }
+    SegHistogramParams hp{};
+    hp.keys            = key_in;
+    hp.counters        = d_counters;
+    hp.tile_slot       = d_tile_slot;
+    hp.slot_first_tile = d_large_first;
+    hp.slot_range      = d_large_range;
+    hp.seg_bins        = d_seg_bins;
+    hp.xor_mask        = xor_mask;
+    hp.large_tile      = uint32_t(large_tile);
+    hp.float_mode      = float_mode;
+    hp.begin_bit       = pr.begin_bit;
+    hp.end_bit         = pr.end_bit;
+    hp.num_passes      = passes;
+    if (hipError_t e = launch_kernel(tab->seg_histogram, tile_bound, kHistThreads, 0, stream, &hp, 5); e != hipSuccess)
+    {
+      return e;
+    }
+  }
+
+  // One-workgroup classes: persistent grids over each class list.
+  const uint64_t cus = uint64_t(device_cu_count());
+  for (int c = 0; c < cfg.num_block_classes; ++c)
+  {
+    BlockSortParams bp{};
+    bp.keys_in        = key_in;
+    bp.keys_out       = key_final;
+    bp.vals_in        = val_in;
+    bp.vals_out       = val_final;
+    bp.seg_range      = static_cast<const int64_t*>(parts[8 + c]);
+    bp.d_num_segments = d_counters + c;
+    bp.num_items      = n;
+    bp.num_segments   = uint64_t(num_segments);
+    bp.xor_mask       = xor_mask;
+    bp.float_mode     = float_mode;
+    bp.begin_bit      = pr.begin_bit;
+    bp.end_bit        = pr.end_bit;
+    const uint64_t per_cu = std::max<uint64_t>(1, (64 * 1024) / std::max<size_t>(cls[c]->lds_bytes, 1) + 1);
+    const uint64_t grid   = std::min<uint64_t>(uint64_t(num_segments), cus * std::min<uint64_t>(per_cu, 8));
+    if (hipError_t e = launch_kernel(cls[c]->fn, grid, cls[c]->threads, cls[c]->lds_bytes, stream, &bp, 3);
+        e != hipSuccess)
+    {
+      return e;
+    }
+  }
+
+  // Large class: one batched onesweep per digit pass.
+  if (large_bound > 0)
+  {
+    for (int pass = 0; pass < passes; ++pass)
+    {
+      const int bit = pr.begin_bit + pass * kRadixBits;
+      OnesweepParams op{};
+      op.keys_in         = pass == 0 ? key_in : key_seq[(pass - 1) & 1];
+      op.keys_out        = key_seq[pass & 1];
+      op.vals_in         = pass == 0 ? val_in : val_seq[(pass - 1) & 1];
+      op.vals_out        = val_seq[pass & 1];
+      op.lookback        = d_lb[pass & 1];
+      op.lookback_next   = d_lb[(pass & 1) ^ 1];
+      op.tile_counter    = d_counters + kSegCounters + pass;
+      op.num_items       = n;
+      op.xor_mask        = xor_mask;
+      op.num_tiles       = uint32_t(tile_bound);
+      op.zero_tiles      = uint32_t(tile_bound);
+      op.float_mode      = float_mode;
+      op.shift           = bit;
+      op.num_bits        = std::max(0, std::min(pr.end_bit - bit, kRadixBits));
+      op.tile_slot       = d_tile_slot;
+      op.slot_first_tile = d_large_first;
+      op.slot_range      = d_large_range;
+      op.seg_bins        = d_seg_bins;
+      op.total_tiles     = d_counters + kSegLargeTiles;
+      op.pass            = pass;
+      op.num_passes      = passes;
+      if (hipError_t e = launch_kernel(large->fn, tile_bound, large->threads, large->lds_bytes, stream, &op, 6);
+          e != hipSuccess)
+      {
+        return e;
+      }
+    }
+  }
+
+  if (pr.overwrite)
+  {
+    keys.selector = (keys.selector + passes) & 1;
+    vals.selector = (vals.selector + passes) & 1;
+  }
+  else
+  {
+    keys.selector = (keys.selector + 1) & 1;
+    vals.selector = (vals.selector + 1) & 1;
+  }
+  return hipSuccess;
+}
+
+} // namespace cccl_amd
+
+extern "C" CCCL_C_API void cccl_amd_segmented_default_config(int key_bytes, int value_bytes, cccl_amd_segmented_config_t* cfg)
+{
+  if (cfg == nullptr)
+  {
+    return;
+  }
+  *cfg                   = cccl_amd_segmented_config_t{};
+  const int item         = key_bytes + value_bytes;
+  cfg->num_block_classes = 4;
+  cfg->block_threads[0]  = 256;
+  cfg->block_ipt[0]      = 4;
+  cfg->block_threads[1]  = 256;
+  cfg->block_ipt[1]      = 8;
+  cfg->block_threads[2]  = 256;
+  cfg->block_ipt[2]      = 16;
+  cfg->block_threads[3]  = 512;
+  cfg->block_ipt[3]      = 16;
+  if (item > 16)
+  {
+    cfg->num_block_classes = 3;
+  }
+  cfg->large_threads = item > 12 ? 256 : 512;
+  cfg->large_ipt     = 16;
+}

@@ -1,0 +1,827 @@
+// gfx950 kernels of the device-wide LSD radix sort (wave64).
+//
+//   init_kernel       zeroes digit bins, tile counters and the first look-back buffer
+//                     (role of DeviceRadixSortInitKernel, kernel_radix_sort.cuh:475-496)
+//   histogram_kernel  one read of the keys builds the digit histogram of every pass
+//                     (role of DeviceRadixSortHistogramKernel, kernel_radix_sort.cuh:447-473,
+//                     agent_radix_sort_histogram.cuh:205-246)
+//   onesweep_kernel   one digit pass: wave-ballot rank, decoupled look-back over tiles,
+//                     LDS-staged scatter (role of DeviceRadixSortOnesweepKernel,
+//                     kernel_radix_sort.cuh:498-558, agent_radix_sort_onesweep.cuh:667-697);
+//                     the per-pass exclusive sum of the bins (ExclusiveSumKernel,
+//                     kernel_radix_sort.cuh:563-606) is folded into every tile
+//   block_sort_kernel whole LSD sort of up to one tile inside one workgroup, one
+//                     workgroup per segment (role of DeviceRadixSortSingleTileKernel,
+//                     kernel_radix_sort.cuh:330-434, and of the per-segment kernel
+//                     kernel_segmented_radix_sort.cuh:100-291 for segments that fit)
+#pragma once
+
+#include <type_traits>
+
+#include "radix_common.hpp"
+
+namespace cccl_amd {
+
+struct InitParams
+{
+  uint64_t* bins;
+  uint64_t num_bins;
+  uint32_t* counters;
+  uint64_t num_counters;
+  uint32_t* lookback;
+  uint64_t num_lookback;
+};
+
+static __global__ void __launch_bounds__(256) init_kernel(InitParams p)
+{
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < p.num_bins; i += stride)
+  {
+    p.bins[i] = 0;
+  }
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < p.num_counters; i += stride)
+  {
+    p.counters[i] = 0;
+  }
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < p.num_lookback; i += stride)
+  {
+    p.lookback[i] = 0;
+  }
+}
+
+struct HistogramParams
+{
+  const void* keys;
+  uint64_t num_items;
+  uint64_t* bins; // [passes][kRadix]
+  uint64_t xor_mask;
+  int float_mode;
+  int begin_bit;
+  int end_bit;
+  int num_passes;
+};
+
+// Sub-histogram copies per block: lanes l and l' with l % kHistParts != l' % kHistParts
+// never touch the same LDS word, which divides same-address atomic conflicts when the
+// key entropy is low.
+constexpr int kHistParts   = 4;
+constexpr int kHistThreads = 256;
+constexpr int kHistUnroll  = 8;
+
+template <class UK>
+__global__ void __launch_bounds__(kHistThreads) histogram_kernel(HistogramParams p)
+{
+  constexpr int kMaxPasses = int(sizeof(UK) * 8 / kRadixBits);
+  __shared__ uint32_t s_hist[kMaxPasses * kRadix * kHistParts];
+
+  for (int i = threadIdx.x; i < kMaxPasses * kRadix * kHistParts; i += kHistThreads)
+  {
+    s_hist[i] = 0;
+  }
+  __syncthreads();
+
+  const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
+  const UK* keys      = static_cast<const UK*>(p.keys);
+  const uint32_t part = threadIdx.x % kHistParts;
+  const int passes    = p.num_passes;
+
+  auto count = [&](UK k) {
+    const UK t = tw(k);
+    for (int pass = 0; pass < passes; ++pass)
+    {
+      const int shift     = p.begin_bit + pass * kRadixBits;
+      const int bits      = min(kRadixBits, p.end_bit - shift);
+      const uint32_t mask = (1u << bits) - 1u;
+      const uint32_t d    = extract_digit(t, shift, mask);
+      __hip_atomic_fetch_add(
+        &s_hist[(pass * kRadix + d) * kHistParts + part], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  };
+
+  const uint64_t n          = p.num_items;
+  const uint64_t block_span = uint64_t(kHistThreads) * kHistUnroll;
+  const uint64_t grid_span  = block_span * gridDim.x;
+  uint64_t base             = uint64_t(blockIdx.x) * block_span;
+  for (; base + block_span <= n; base += grid_span)
+  {
+    UK k[kHistUnroll];
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u)
+    {
+      k[u] = keys[base + uint64_t(u) * kHistThreads + threadIdx.x];
+    }
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u)
+    {
+      count(k[u]);
+    }
+  }
+  if (base < n)
+  {
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u)
+    {
+      const uint64_t i = base + uint64_t(u) * kHistThreads + threadIdx.x;
+      if (i < n)
+      {
+        count(keys[i]);
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int i = threadIdx.x; i < passes * kRadix; i += kHistThreads)
+  {
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < kHistParts; ++q)
+    {
+      s += s_hist[i * kHistParts + q];
+    }
+    if (s != 0)
+    {
+      __hip_atomic_fetch_add(&p.bins[i], uint64_t(s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// One onesweep launch covers either one portion of a plain sort (SEG == false: tile t
+// holds items [t*TILE, (t+1)*TILE) of the portion) or every large segment of a batched
+// segmented sort at once (SEG == true: tiles are grouped per segment, the tile range of
+// segment slot s starts at slot_first_tile[s], and look-back stops at the segment's
+// first tile).
+struct OnesweepParams
+{
+  const void* keys_in; // plain: first item of this portion; SEG: whole array
+  void* keys_out; // whole output array
+  const void* vals_in;
+  void* vals_out;
+  uint32_t* lookback; // [tiles][kRadix], zero on entry
+  uint32_t* lookback_next; // cleared here for the next launch (may be null)
+  uint32_t* tile_counter; // zero on entry
+  const uint64_t* bins; // plain: digit counts of the whole input (bins_are_starts == 0) or digit starts of this portion
+  uint64_t* bins_next; // plain: last tile writes digit starts of the next portion (may be null)
+  uint64_t num_items; // plain: items in this portion (< 2^30)
+  uint64_t xor_mask;
+  uint32_t num_tiles; // tiles of this launch (SEG: the grid size, an upper bound)
+  uint32_t zero_tiles; // tiles of lookback_next to clear
+  int float_mode;
+  int shift;
+  int num_bits;
+  int bins_are_starts;
+  // SEG only
+  const uint32_t* tile_slot; // segment slot of each tile
+  const uint32_t* slot_first_tile; // first tile of each slot
+  const int64_t* slot_range; // [slot][2] = begin, end item offsets
+  const uint32_t* seg_bins; // [slot][num_passes][kRadix] digit counts
+  const uint32_t* total_tiles; // number of real tiles (device)
+  int pass;
+  int num_passes;
+};
+
+template <class UK, class V, int THREADS, int IPT>
+struct OnesweepLayout
+{
+  static constexpr int kWaves      = THREADS / 64;
+  static constexpr int kTile       = THREADS * IPT;
+  static constexpr bool kHasV      = !std::is_same<V, NoValue>::value;
+  static constexpr size_t kWcnt    = size_t(kWaves) * kRadix * 4;
+  static constexpr size_t kBase    = size_t(kRadix) * 8;
+  static constexpr size_t kMisc    = 64;
+  static constexpr size_t kKeys    = size_t(kTile) * sizeof(UK);
+  static constexpr size_t kVals    = kHasV ? size_t(kTile) * sizeof(V) : 0;
+  static constexpr size_t kOffBase = kWcnt;
+  static constexpr size_t kOffMisc = kOffBase + kBase;
+  static constexpr size_t kOffKeys = kOffMisc + kMisc;
+  static constexpr size_t kOffVals = (kOffKeys + kKeys + 15) & ~size_t(15);
+  static constexpr size_t kBytes   = kOffVals + kVals;
+};
+
+template <class UK, class V, int THREADS, int IPT, bool SEG>
+__global__ void __launch_bounds__(THREADS) onesweep_kernel(OnesweepParams p)
+{
+  using L              = OnesweepLayout<UK, V, THREADS, IPT>;
+  constexpr int kWaves = L::kWaves;
+  constexpr int kTile  = L::kTile;
+  constexpr bool kHasV = L::kHasV;
+  static_assert(THREADS >= kRadix, "one thread per digit is required");
+  using VS = typename std::conditional<kHasV, V, uint8_t>::type;
+
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t* s_wcnt = reinterpret_cast<uint32_t*>(smem);
+  uint64_t* s_base = reinterpret_cast<uint64_t*>(smem + L::kOffBase);
+  uint32_t* s_misc = reinterpret_cast<uint32_t*>(smem + L::kOffMisc);
+  UK* s_keys       = reinterpret_cast<UK*>(smem + L::kOffKeys);
+  VS* s_vals       = reinterpret_cast<VS*>(smem + L::kOffVals);
+
+  const uint32_t tid  = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t wave = tid >> 6;
+
+  for (uint32_t i = tid; i < uint32_t(kWaves * kRadix); i += THREADS)
+  {
+    s_wcnt[i] = 0;
+  }
+  if (tid == 0)
+  {
+    s_misc[0] = __hip_atomic_fetch_add(p.tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const uint32_t tile = s_misc[0];
+
+  // Clear this tile's entries of the other look-back buffer for the next launch; that
+  // buffer is not read by this launch.
+  if (p.lookback_next != nullptr)
+  {
+    for (uint64_t t = tile; t < p.zero_tiles; t += p.num_tiles)
+    {
+      for (uint32_t i = tid; i < uint32_t(kRadix); i += THREADS)
+      {
+        p.lookback_next[t * kRadix + i] = 0u;
+      }
+    }
+  }
+
+  uint64_t tile_base  = 0; // first item of the tile in keys_in
+  uint64_t left       = 0;
+  uint32_t first_tile = 0; // tile whose look-back needs no predecessor
+  uint32_t slot       = 0;
+  uint64_t seg_begin  = 0;
+  if constexpr (SEG)
+  {
+    if (tile >= *p.total_tiles)
+    {
+      return; // block-uniform: the grid is an upper bound on the tile count
+    }
+    slot                  = p.tile_slot[tile];
+    first_tile            = p.slot_first_tile[slot];
+    seg_begin             = uint64_t(p.slot_range[2 * slot]);
+    const uint64_t seg_end = uint64_t(p.slot_range[2 * slot + 1]);
+    tile_base             = seg_begin + uint64_t(tile - first_tile) * kTile;
+    left                  = seg_end - tile_base;
+  }
+  else
+  {
+    tile_base = uint64_t(tile) * kTile;
+    left      = p.num_items - tile_base;
+  }
+  const uint32_t valid = left < uint64_t(kTile) ? uint32_t(left) : uint32_t(kTile);
+  const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
+  const uint32_t dmask = (1u << p.num_bits) - 1u;
+
+  // Load: wave w owns the strip [w*IPT*64, (w+1)*IPT*64) of the tile; item i of lane l
+  // is strip element i*64 + l, so each load instruction reads 64 consecutive items.
+  const uint32_t strip = wave * (IPT * 64) + lane;
+  const UK* kin        = static_cast<const UK*>(p.keys_in) + tile_base + strip;
+  const VS* vin        = static_cast<const VS*>(p.vals_in) + tile_base + strip;
+  UK keys[IPT];
+  VS vals[IPT];
+  uint32_t dig[IPT];
+  if (valid == uint32_t(kTile))
+  {
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      keys[i] = kin[i * 64];
+    }
+    if constexpr (kHasV)
+    {
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        vals[i] = vin[i * 64];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      dig[i] = extract_digit(tw(keys[i]), p.shift, dmask);
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      const bool ok = strip + uint32_t(i * 64) < valid;
+      keys[i]       = ok ? kin[i * 64] : UK(0);
+      if constexpr (kHasV)
+      {
+        if (ok)
+        {
+          vals[i] = vin[i * 64];
+        }
+      }
+      // items past the end take the largest digit; being last in input order they
+      // rank after every real item
+      dig[i] = ok ? extract_digit(tw(keys[i]), p.shift, dmask) : uint32_t(kRadix - 1);
+    }
+  }
+
+  // Rank within the wave.
+  uint32_t rank[IPT];
+  wave_rank<IPT>(dig, rank, s_wcnt + wave * kRadix);
+  __syncthreads();
+
+  // Thread d (< kRadix) owns digit d: per-wave exclusive offsets and the tile count.
+  uint32_t total = 0;
+  if (tid < uint32_t(kRadix))
+  {
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w)
+    {
+      const uint32_t c         = s_wcnt[w * kRadix + tid];
+      s_wcnt[w * kRadix + tid] = total;
+      total += c;
+    }
+  }
+  const uint32_t pad         = uint32_t(kTile) - valid;
+  const uint32_t valid_total = (tid == uint32_t(kRadix - 1)) ? total - pad : total;
+  if (tid < uint32_t(kRadix))
+  {
+    store_status(&p.lookback[uint64_t(tile) * kRadix + tid],
+                 (tile == first_tile ? kFlagGlobal : kFlagPartial) | valid_total);
+  }
+
+  uint32_t tile_items  = 0;
+  const uint32_t start = radix_exclusive_scan<uint32_t>(total, s_misc + 4, tile_items);
+  uint64_t gstart      = 0;
+  {
+    uint64_t* s_scan64 = reinterpret_cast<uint64_t*>(s_misc + 8);
+    uint64_t all       = 0;
+    if constexpr (SEG)
+    {
+      const uint32_t* sb = p.seg_bins + (uint64_t(slot) * p.num_passes + p.pass) * kRadix;
+      gstart = seg_begin + radix_exclusive_scan<uint64_t>(tid < uint32_t(kRadix) ? uint64_t(sb[tid]) : 0, s_scan64, all);
+    }
+    else if (p.bins_are_starts)
+    {
+      gstart = tid < uint32_t(kRadix) ? p.bins[tid] : 0;
+    }
+    else
+    {
+      gstart = radix_exclusive_scan<uint64_t>(tid < uint32_t(kRadix) ? p.bins[tid] : 0, s_scan64, all);
+    }
+  }
+  if (tid < uint32_t(kRadix))
+  {
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w)
+    {
+      s_wcnt[w * kRadix + tid] += start;
+    }
+  }
+  __syncthreads();
+
+  // Stage the tile in LDS in digit order.
+  {
+    const uint32_t* wc = s_wcnt + wave * kRadix;
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      const uint32_t pos = wc[dig[i]] + rank[i];
+      s_keys[pos]        = keys[i];
+      if constexpr (kHasV)
+      {
+        s_vals[pos] = vals[i];
+      }
+    }
+  }
+
+  // Decoupled look-back: walk earlier tiles, adding their counts, until one carries
+  // the inclusive prefix.
+  if (tid < uint32_t(kRadix))
+  {
+    uint32_t excl = 0;
+    if (tile > first_tile)
+    {
+      int64_t t = int64_t(tile) - 1;
+      while (true)
+      {
+        const uint32_t* w = &p.lookback[uint64_t(t) * kRadix + tid];
+        uint32_t v        = load_status(w);
+        while (v == 0u)
+        {
+          __builtin_amdgcn_s_sleep(1);
+          v = load_status(w);
+        }
+        excl += v & kCountMask;
+        if (v & kFlagGlobal)
+        {
+          break;
+        }
+        --t;
+      }
+      store_status(&p.lookback[uint64_t(tile) * kRadix + tid], kFlagGlobal | (excl + valid_total));
+    }
+    s_base[tid] = gstart + excl - start;
+    if constexpr (!SEG)
+    {
+      if (p.bins_next != nullptr && tile == p.num_tiles - 1u)
+      {
+        p.bins_next[tid] = gstart + excl + valid_total;
+      }
+    }
+  }
+  __syncthreads();
+
+  // Write out in LDS order: consecutive threads take consecutive staged items, which
+  // land at consecutive addresses within each digit's run.
+  UK* kout = static_cast<UK*>(p.keys_out);
+  VS* vout = static_cast<VS*>(p.vals_out);
+#pragma unroll
+  for (int i = 0; i < IPT; ++i)
+  {
+    const uint32_t pos = uint32_t(i) * THREADS + tid;
+    if (pos < valid)
+    {
+      const UK k         = s_keys[pos];
+      const uint32_t d   = extract_digit(tw(k), p.shift, dmask);
+      const uint64_t dst = s_base[d] + pos;
+      kout[dst]          = k;
+      if constexpr (kHasV)
+      {
+        vout[dst] = s_vals[pos];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Whole sort of one segment (<= one tile) inside one workgroup.
+// ---------------------------------------------------------------------------
+
+struct BlockSortParams
+{
+  const void* keys_in;
+  void* keys_out;
+  const void* vals_in;
+  void* vals_out;
+  const int64_t* seg_range; // [s][2] = begin, end; null: a single segment [0, num_items)
+  const uint32_t* d_num_segments; // segment count in device memory (null: num_segments)
+  uint64_t num_items;
+  uint64_t num_segments;
+  uint64_t xor_mask;
+  int float_mode;
+  int begin_bit;
+  int end_bit;
+  int pad_;
+};
+
+template <class UK, class V, int THREADS, int IPT>
+struct BlockSortLayout
+{
+  static constexpr int kWaves      = THREADS / 64;
+  static constexpr int kTile       = THREADS * IPT;
+  static constexpr bool kHasV      = !std::is_same<V, NoValue>::value;
+  static constexpr size_t kWcnt    = size_t(kWaves) * kRadix * 4;
+  static constexpr size_t kMisc    = 64;
+  static constexpr size_t kKeys    = size_t(kTile) * sizeof(UK);
+  static constexpr size_t kVals    = kHasV ? size_t(kTile) * sizeof(V) : 0;
+  static constexpr size_t kOffMisc = kWcnt;
+  static constexpr size_t kOffKeys = kOffMisc + kMisc;
+  static constexpr size_t kOffVals = (kOffKeys + kKeys + 15) & ~size_t(15);
+  static constexpr size_t kBytes   = kOffVals + kVals;
+};
+
+// Segments are independent and each is read whole into registers before any item of
+// it is written, so keys_out may equal keys_in (in-place) as long as segments do not
+// overlap.
+template <class UK, class V, int THREADS, int IPT>
+__global__ void __launch_bounds__(THREADS) block_sort_kernel(BlockSortParams p)
+{
+  using L              = BlockSortLayout<UK, V, THREADS, IPT>;
+  constexpr int kWaves = L::kWaves;
+  constexpr bool kHasV = L::kHasV;
+  static_assert(THREADS >= kRadix, "one thread per digit is required");
+  using VS = typename std::conditional<kHasV, V, uint8_t>::type;
+
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t* s_wcnt = reinterpret_cast<uint32_t*>(smem);
+  uint32_t* s_misc = reinterpret_cast<uint32_t*>(smem + L::kOffMisc);
+  UK* s_keys       = reinterpret_cast<UK*>(smem + L::kOffKeys);
+  VS* s_vals       = reinterpret_cast<VS*>(smem + L::kOffVals);
+
+  const uint32_t tid   = threadIdx.x;
+  const uint32_t lane  = tid & 63u;
+  const uint32_t wave  = tid >> 6;
+  const uint32_t strip = wave * (IPT * 64) + lane;
+  const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
+  const uint64_t num_segments = p.d_num_segments != nullptr ? uint64_t(*p.d_num_segments) : p.num_segments;
+
+  for (uint64_t s = blockIdx.x; s < num_segments; s += gridDim.x)
+  {
+    const uint64_t b = p.seg_range != nullptr ? uint64_t(p.seg_range[2 * s]) : 0;
+    const uint64_t e = p.seg_range != nullptr ? uint64_t(p.seg_range[2 * s + 1]) : p.num_items;
+    if (e <= b)
+    {
+      continue; // block-uniform
+    }
+    const uint32_t m = uint32_t(e - b);
+
+    const UK* kin = static_cast<const UK*>(p.keys_in) + b;
+    const VS* vin = static_cast<const VS*>(p.vals_in) + b;
+    UK keys[IPT];
+    VS vals[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      const uint32_t j = strip + uint32_t(i * 64);
+      keys[i]          = j < m ? kin[j] : UK(0);
+      if constexpr (kHasV)
+      {
+        if (j < m)
+        {
+          vals[i] = vin[j];
+        }
+      }
+    }
+    // Every item of the segment is in registers before the block writes anything.
+    __syncthreads();
+
+    for (int shift = p.begin_bit; shift < p.end_bit; shift += kRadixBits)
+    {
+      const int bits       = min(kRadixBits, p.end_bit - shift);
+      const uint32_t dmask = (1u << bits) - 1u;
+      uint32_t dig[IPT];
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        const uint32_t j = strip + uint32_t(i * 64);
+        dig[i]           = j < m ? extract_digit(tw(keys[i]), shift, dmask) : uint32_t(kRadix - 1);
+      }
+      for (uint32_t i = tid; i < uint32_t(kWaves * kRadix); i += THREADS)
+      {
+        s_wcnt[i] = 0;
+      }
+      __syncthreads();
+
+      uint32_t rank[IPT];
+      wave_rank<IPT>(dig, rank, s_wcnt + wave * kRadix);
+      __syncthreads();
+
+      uint32_t total = 0;
+      if (tid < uint32_t(kRadix))
+      {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w)
+        {
+          const uint32_t c         = s_wcnt[w * kRadix + tid];
+          s_wcnt[w * kRadix + tid] = total;
+          total += c;
+        }
+      }
+      uint32_t all         = 0;
+      const uint32_t start = radix_exclusive_scan<uint32_t>(total, s_misc, all);
+      if (tid < uint32_t(kRadix))
+      {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w)
+        {
+          s_wcnt[w * kRadix + tid] += start;
+        }
+      }
+      __syncthreads();
+
+      const uint32_t* wc = s_wcnt + wave * kRadix;
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        const uint32_t pos = wc[dig[i]] + rank[i];
+        s_keys[pos]        = keys[i];
+        if constexpr (kHasV)
+        {
+          s_vals[pos] = vals[i];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        const uint32_t j = strip + uint32_t(i * 64);
+        keys[i]          = s_keys[j];
+        if constexpr (kHasV)
+        {
+          vals[i] = s_vals[j];
+        }
+      }
+      __syncthreads();
+    }
+
+    UK* kout = static_cast<UK*>(p.keys_out) + b;
+    VS* vout = static_cast<VS*>(p.vals_out) + b;
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      const uint32_t j = strip + uint32_t(i * 64);
+      if (j < m)
+      {
+        kout[j] = keys[i];
+        if constexpr (kHasV)
+        {
+          vout[j] = vals[i];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched segmented sort: size binning and per-segment histograms.
+// ---------------------------------------------------------------------------
+
+// Segments are binned by size into up to kMaxBlockClasses one-workgroup classes
+// (class c takes sizes in (max[c-1], max[c]]) and one large class (sizes above the last
+// class max) that the batched onesweep sorts.
+constexpr int kMaxBlockClasses = 4;
+
+enum SegCounter : int
+{
+  kSegLarge      = kMaxBlockClasses, // counters[0..kMaxBlockClasses) count each class
+  kSegLargeTiles = kMaxBlockClasses + 1,
+  kSegCounters   = kMaxBlockClasses + 2,
+};
+
+struct SegInitParams
+{
+  uint32_t* counters; // kSegCounters + passes tile counters
+  uint32_t num_counters;
+  uint32_t* lookback;
+  uint64_t num_lookback;
+};
+
+static __global__ void __launch_bounds__(256) seg_init_kernel(SegInitParams p)
+{
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  const uint64_t i0     = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i0 < p.num_counters)
+  {
+    p.counters[i0] = 0;
+  }
+  for (uint64_t i = i0; i < p.num_lookback; i += stride)
+  {
+    p.lookback[i] = 0;
+  }
+}
+
+struct SegClassifyParams
+{
+  const void* begin_offsets;
+  const void* end_offsets;
+  int64_t num_segments;
+  int offset_bytes;
+  int num_classes;
+  uint32_t class_max[kMaxBlockClasses]; // largest segment of each one-workgroup class
+  uint32_t large_tile; // onesweep tile of the large class
+  uint32_t* counters;
+  int64_t* class_range[kMaxBlockClasses]; // [s][2] = begin, end
+  int64_t* large_range;
+  uint32_t* large_first_tile;
+};
+
+// One thread per segment.  List order inside a class follows the atomics and is not
+// deterministic; every segment is sorted independently, so results do not depend on it.
+static __global__ void __launch_bounds__(256) seg_classify_kernel(SegClassifyParams p)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= p.num_segments)
+  {
+    return;
+  }
+  int64_t b, e;
+  if (p.offset_bytes == 4)
+  {
+    b = static_cast<const int32_t*>(p.begin_offsets)[i];
+    e = static_cast<const int32_t*>(p.end_offsets)[i];
+  }
+  else
+  {
+    b = static_cast<const int64_t*>(p.begin_offsets)[i];
+    e = static_cast<const int64_t*>(p.end_offsets)[i];
+  }
+  if (e <= b)
+  {
+    return;
+  }
+  const uint64_t size = uint64_t(e - b);
+  for (int c = 0; c < p.num_classes; ++c)
+  {
+    if (size <= p.class_max[c])
+    {
+      const uint32_t s           = atomicAdd(&p.counters[c], 1u);
+      p.class_range[c][2 * s]     = b;
+      p.class_range[c][2 * s + 1] = e;
+      return;
+    }
+  }
+  const uint32_t s         = atomicAdd(&p.counters[kSegLarge], 1u);
+  const uint32_t tiles     = uint32_t((size + p.large_tile - 1) / p.large_tile);
+  p.large_range[2 * s]     = b;
+  p.large_range[2 * s + 1] = e;
+  p.large_first_tile[s]    = atomicAdd(&p.counters[kSegLargeTiles], tiles);
+}
+
+struct SegPrepParams
+{
+  const uint32_t* counters;
+  const int64_t* large_range;
+  const uint32_t* large_first_tile;
+  uint32_t* tile_slot;
+  uint32_t* seg_bins; // [slot][passes][kRadix]
+  uint32_t large_tile;
+  int num_passes;
+};
+
+// One workgroup per large-segment slot (the grid is an upper bound on their number):
+// clears the slot's digit counts and writes the slot id of each of its tiles.
+static __global__ void __launch_bounds__(256) seg_prep_kernel(SegPrepParams p)
+{
+  const uint32_t slot = blockIdx.x;
+  if (slot >= p.counters[kSegLarge])
+  {
+    return;
+  }
+  uint32_t* bins = p.seg_bins + uint64_t(slot) * p.num_passes * kRadix;
+  for (int i = threadIdx.x; i < p.num_passes * kRadix; i += blockDim.x)
+  {
+    bins[i] = 0;
+  }
+  const uint64_t size  = uint64_t(p.large_range[2 * slot + 1] - p.large_range[2 * slot]);
+  const uint32_t tiles = uint32_t((size + p.large_tile - 1) / p.large_tile);
+  const uint32_t first = p.large_first_tile[slot];
+  for (uint32_t t = threadIdx.x; t < tiles; t += blockDim.x)
+  {
+    p.tile_slot[first + t] = slot;
+  }
+}
+
+struct SegHistogramParams
+{
+  const void* keys;
+  const uint32_t* counters;
+  const uint32_t* tile_slot;
+  const uint32_t* slot_first_tile;
+  const int64_t* slot_range;
+  uint32_t* seg_bins;
+  uint64_t xor_mask;
+  uint32_t large_tile;
+  int float_mode;
+  int begin_bit;
+  int end_bit;
+  int num_passes;
+};
+
+// One workgroup per large tile (grid = upper bound): digit counts of every pass for the
+// tile's items, added into its segment's counts.
+template <class UK>
+__global__ void __launch_bounds__(kHistThreads) seg_histogram_kernel(SegHistogramParams p)
+{
+  constexpr int kMaxPasses = int(sizeof(UK) * 8 / kRadixBits);
+  __shared__ uint32_t s_hist[kMaxPasses * kRadix * kHistParts];
+  const uint32_t tile = blockIdx.x;
+  if (tile >= p.counters[kSegLargeTiles])
+  {
+    return;
+  }
+  for (int i = threadIdx.x; i < kMaxPasses * kRadix * kHistParts; i += kHistThreads)
+  {
+    s_hist[i] = 0;
+  }
+  __syncthreads();
+  const uint32_t slot    = p.tile_slot[tile];
+  const uint64_t b       = uint64_t(p.slot_range[2 * slot]) + uint64_t(tile - p.slot_first_tile[slot]) * p.large_tile;
+  const uint64_t e       = min(uint64_t(p.slot_range[2 * slot + 1]), b + p.large_tile);
+  const UK* keys         = static_cast<const UK*>(p.keys);
+  const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
+  const uint32_t part = threadIdx.x % kHistParts;
+  for (uint64_t i = b + threadIdx.x; i < e; i += kHistThreads)
+  {
+    const UK t = tw(keys[i]);
+    for (int pass = 0; pass < p.num_passes; ++pass)
+    {
+      const int shift     = p.begin_bit + pass * kRadixBits;
+      const int bits      = min(kRadixBits, p.end_bit - shift);
+      const uint32_t mask = bits > 0 ? (1u << bits) - 1u : 0u;
+      const uint32_t d    = extract_digit(t, shift, mask);
+      __hip_atomic_fetch_add(
+        &s_hist[(pass * kRadix + d) * kHistParts + part], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  __syncthreads();
+  uint32_t* bins = p.seg_bins + uint64_t(slot) * p.num_passes * kRadix;
+  for (int i = threadIdx.x; i < p.num_passes * kRadix; i += kHistThreads)
+  {
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < kHistParts; ++q)
+    {
+      s += s_hist[i * kHistParts + q];
+    }
+    if (s != 0)
+    {
+      __hip_atomic_fetch_add(&bins[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+} // namespace cccl_amd

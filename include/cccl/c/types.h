@@ -1,0 +1,182 @@
+/* Shared C-ABI types of the cccl_amd radix sort.
+ *
+ * Replaces c/parallel/include/cccl/c/types.h:30-185 (same enum names and values,
+ * same struct field order) so that a caller compiled against the reference
+ * header links against libcccl_amd.so without source changes.
+ *
+ * The reference takes CUresult / CUstream / CUlibrary / CUkernel from <cuda.h>.
+ * On MI355X there is no CUDA driver, so this header defines them itself:
+ *   CUresult  - an int-valued enum whose codes equal hipError_t's for the codes the
+ *               radix sort returns (0 success, 1 invalid value, 2 out of memory,
+ *               801 not supported, 999 unknown), so a HIP error casts through.
+ *   CUstream  - hipStream_t (struct ihipStream_t*).
+ *   CUlibrary - opaque handle to this library's table of gfx950 code objects.
+ *   CUkernel  - opaque handle to one entry of that table.
+ * Define CCCL_AMD_HAVE_CU_TYPES before including to supply them yourself. */
+#pragma once
+
+#ifndef CCCL_C_EXPERIMENTAL
+#  error "C exposure is experimental and subject to change. Define CCCL_C_EXPERIMENTAL to acknowledge this notice."
+#endif
+
+#if defined(_WIN32)
+#  define CCCL_C_API __declspec(dllexport)
+#else
+#  define CCCL_C_API __attribute__((__visibility__("default")))
+#endif
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <cccl/c/extern_c.h>
+
+CCCL_C_EXTERN_C_BEGIN
+
+#ifndef CCCL_AMD_HAVE_CU_TYPES
+typedef enum cudaError_enum
+{
+#  ifndef CUDA_SUCCESS /* hip_runtime.h defines it as hipSuccess (0) */
+  CUDA_SUCCESS = 0,
+#  endif
+  CUDA_ERROR_INVALID_VALUE   = 1,
+  CUDA_ERROR_OUT_OF_MEMORY   = 2,
+  CUDA_ERROR_NOT_INITIALIZED = 3,
+  CUDA_ERROR_INVALID_DEVICE  = 101,
+  CUDA_ERROR_INVALID_IMAGE   = 200,
+  CUDA_ERROR_INVALID_HANDLE  = 400,
+  CUDA_ERROR_NOT_FOUND       = 500,
+  CUDA_ERROR_NOT_SUPPORTED   = 801,
+  CUDA_ERROR_UNKNOWN         = 999
+} CUresult;
+typedef struct ihipStream_t* CUstream;
+typedef struct cccl_amd_library_st* CUlibrary;
+typedef struct cccl_amd_kernel_st* CUkernel;
+#endif
+
+typedef enum cccl_type_enum
+{
+  CCCL_INT8    = 0,
+  CCCL_INT16   = 1,
+  CCCL_INT32   = 2,
+  CCCL_INT64   = 3,
+  CCCL_UINT8   = 4,
+  CCCL_UINT16  = 5,
+  CCCL_UINT32  = 6,
+  CCCL_UINT64  = 7,
+  CCCL_FLOAT16 = 8,
+  CCCL_FLOAT32 = 9,
+  CCCL_FLOAT64 = 10,
+  CCCL_STORAGE = 11,
+  CCCL_BOOLEAN = 12,
+} cccl_type_enum;
+
+typedef struct cccl_type_info
+{
+  size_t size;
+  size_t alignment;
+  cccl_type_enum type;
+} cccl_type_info;
+
+typedef enum cccl_op_kind_t
+{
+  CCCL_STATELESS     = 0,
+  CCCL_STATEFUL      = 1,
+  CCCL_PLUS          = 2,
+  CCCL_MINUS         = 3,
+  CCCL_MULTIPLIES    = 4,
+  CCCL_DIVIDES       = 5,
+  CCCL_MODULUS       = 6,
+  CCCL_EQUAL_TO      = 7,
+  CCCL_NOT_EQUAL_TO  = 8,
+  CCCL_GREATER       = 9,
+  CCCL_LESS          = 10,
+  CCCL_GREATER_EQUAL = 11,
+  CCCL_LESS_EQUAL    = 12,
+  CCCL_LOGICAL_AND   = 13,
+  CCCL_LOGICAL_OR    = 14,
+  CCCL_LOGICAL_NOT   = 15,
+  CCCL_BIT_AND       = 16,
+  CCCL_BIT_OR        = 17,
+  CCCL_BIT_XOR       = 18,
+  CCCL_BIT_NOT       = 19,
+  CCCL_IDENTITY      = 20,
+  CCCL_NEGATE        = 21,
+  CCCL_MINIMUM       = 22,
+  CCCL_MAXIMUM       = 23,
+} cccl_op_kind_t;
+
+typedef enum cccl_op_code_type
+{
+  CCCL_OP_LTOIR      = 0,
+  CCCL_OP_CPP_SOURCE = 1
+} cccl_op_code_type;
+
+typedef struct cccl_op_t
+{
+  cccl_op_kind_t type;
+  const char* name;
+  const char* code;
+  size_t code_size;
+  cccl_op_code_type code_type;
+  size_t size;
+  size_t alignment;
+  void* state;
+  const char** extra_ltoirs;
+  size_t* extra_ltoir_sizes;
+  size_t num_extra_ltoirs;
+  cccl_op_code_type* extra_code_types;
+} cccl_op_t;
+
+typedef struct cccl_build_config
+{
+  const char** extra_compile_flags;
+  size_t num_extra_compile_flags;
+  const char** extra_include_dirs;
+  size_t num_extra_include_dirs;
+} cccl_build_config;
+
+typedef enum cccl_iterator_kind_t
+{
+  CCCL_POINTER  = 0,
+  CCCL_ITERATOR = 1,
+} cccl_iterator_kind_t;
+
+typedef struct cccl_value_t
+{
+  cccl_type_info type;
+  void* state;
+} cccl_value_t;
+
+typedef union
+{
+  int64_t signed_offset;
+  uint64_t unsigned_offset;
+} cccl_increment_t;
+
+typedef void (*cccl_host_op_fn_ptr_t)(void*, cccl_increment_t);
+
+typedef struct cccl_iterator_t
+{
+  size_t size;
+  size_t alignment;
+  cccl_iterator_kind_t type;
+  cccl_op_t advance;
+  cccl_op_t dereference;
+  cccl_type_info value_type;
+  void* state;
+  cccl_host_op_fn_ptr_t host_advance;
+} cccl_iterator_t;
+
+typedef enum cccl_sort_order_t
+{
+  CCCL_ASCENDING  = 0,
+  CCCL_DESCENDING = 1,
+} cccl_sort_order_t;
+
+typedef enum cccl_payload_kind_t
+{
+  CCCL_PAYLOAD_LTOIR = 0,
+  CCCL_PAYLOAD_CUBIN = 1,
+} cccl_payload_kind_t;
+
+CCCL_C_EXTERN_C_END

@@ -1,0 +1,126 @@
+/* Typed C entry points of the MI355X radix sort, below the cccl_device_radix_sort_*
+ * C-ABI.  The C++ API (cccl_amd/device_radix_sort.hpp) and the Python binding call
+ * these; they carry the launch configuration explicitly so the autotuner and the
+ * config selector can pick it per call.
+ *
+ * Return values are hipError_t codes (0 = success, 1 = invalid value, 2 = out of
+ * memory, 801 = not supported).  Every entry point is stream-ordered: it enqueues
+ * kernels on `stream` and returns without synchronising. */
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifndef CCCL_C_EXPERIMENTAL
+#  define CCCL_C_EXPERIMENTAL
+#endif
+#include <cccl/c/types.h>
+
+CCCL_C_EXTERN_C_BEGIN
+
+/* Launch configuration of the device-wide sort.  Plays the role of CUB's
+ * RadixSortPolicy (cub/cub/device/dispatch/tuning/tuning_radix_sort.cuh:255-293):
+ * onesweep tile = onesweep_threads * onesweep_ipt items; inputs of at most
+ * single_tile_threads * single_tile_ipt items are sorted by one workgroup. */
+typedef struct cccl_amd_radix_sort_config_t
+{
+  int32_t onesweep_threads;
+  int32_t onesweep_ipt;
+  int32_t radix_bits; /* digit width; 8 is the only compiled width */
+  int32_t single_tile_threads; /* 0 disables the one-workgroup path */
+  int32_t single_tile_ipt;
+  int32_t histogram_blocks_per_cu;
+  int32_t reserved[2];
+} cccl_amd_radix_sort_config_t;
+
+/* Launch configuration of the batched segmented sort.  Segments are binned by size:
+ * one-workgroup class c (c < num_block_classes, tiles increasing with c) takes the
+ * segments of more than block_threads[c-1]*block_ipt[c-1] and at most
+ * block_threads[c]*block_ipt[c] items, each sorted whole by one workgroup; larger
+ * segments are sorted together by one batched onesweep launch per digit pass with
+ * tiles of large_threads*large_ipt items. */
+typedef struct cccl_amd_segmented_config_t
+{
+  int32_t num_block_classes; /* 0..4 */
+  int32_t block_threads[4];
+  int32_t block_ipt[4];
+  int32_t large_threads;
+  int32_t large_ipt;
+  int32_t reserved[2];
+} cccl_amd_segmented_config_t;
+
+/* Device-wide stable LSD sort.  Replaces cub::DeviceRadixSort::Sort{Keys,Pairs}
+ * [Descending] (cub/cub/device/device_radix_sort.cuh:411-449, :1126-1154, :1775, :3034,
+ * :4211) and its dispatch (cub/cub/device/dispatch/dispatch_radix_sort.cuh:2008-2065).
+ * keys0/keys1 with *selector form a cub::DoubleBuffer; the pointer API is
+ * is_overwrite_okay = 0 with keys0 = input and keys1 = output.  vals0 == NULL sorts keys
+ * only.  value_bytes must be 1, 2, 4, 8 or 16 (or 0 for keys only).  cfg == NULL picks
+ * the built-in default.  On return *selector names the buffer holding the result. */
+CCCL_C_API int cccl_amd_radix_sort(
+  void* d_temp_storage,
+  size_t* temp_storage_bytes,
+  void* keys0,
+  void* keys1,
+  void* vals0,
+  void* vals1,
+  int* selector,
+  cccl_type_enum key_type,
+  size_t value_bytes,
+  uint64_t num_items,
+  int begin_bit,
+  int end_bit,
+  int descending,
+  int is_overwrite_okay,
+  const cccl_amd_radix_sort_config_t* cfg,
+  CUstream stream);
+
+/* Batched segmented stable sort.  Replaces cub::DeviceSegmentedRadixSort::Sort{Keys,Pairs}
+ * [Descending] (cub/cub/device/device_segmented_radix_sort.cuh:233-270, :417, :1531, :2139)
+ * and its dispatch (cub/cub/device/dispatch/dispatch_segmented_radix_sort.cuh:678-979).
+ * Segment i is [d_begin_offsets[i], d_end_offsets[i]); offsets are int32 (offset_bytes 4)
+ * or int64 (offset_bytes 8); empty segments are allowed; items outside every segment
+ * are not written.  Same buffer and selector rules as cccl_amd_radix_sort. */
+CCCL_C_API int cccl_amd_segmented_radix_sort(
+  void* d_temp_storage,
+  size_t* temp_storage_bytes,
+  void* keys0,
+  void* keys1,
+  void* vals0,
+  void* vals1,
+  int* selector,
+  cccl_type_enum key_type,
+  size_t value_bytes,
+  uint64_t num_items,
+  int64_t num_segments,
+  const void* d_begin_offsets,
+  const void* d_end_offsets,
+  int offset_bytes,
+  int begin_bit,
+  int end_bit,
+  int descending,
+  int is_overwrite_okay,
+  const cccl_amd_segmented_config_t* cfg,
+  CUstream stream);
+
+/* Default configuration for a problem shape (what cfg == NULL uses). */
+CCCL_C_API void cccl_amd_radix_sort_default_config(
+  int key_bytes, int value_bytes, uint64_t num_items, cccl_amd_radix_sort_config_t* cfg);
+CCCL_C_API void cccl_amd_segmented_default_config(int key_bytes, int value_bytes, cccl_amd_segmented_config_t* cfg);
+
+/* 1 when the shapes in cfg were compiled for these widths and fit in LDS. */
+CCCL_C_API int cccl_amd_radix_sort_config_valid(const cccl_amd_radix_sort_config_t* cfg, int key_bytes, int value_bytes);
+
+/* Compiled onesweep / one-workgroup shapes: writes up to max (threads, ipt) pairs and
+ * returns how many exist. */
+CCCL_C_API int cccl_amd_onesweep_shapes(int* threads, int* ipt, int max);
+CCCL_C_API int cccl_amd_block_sort_shapes(int* threads, int* ipt, int max);
+
+/* Kernel timing for benchmarks: while enabled, every kernel the sorts launch is
+ * bracketed by a pair of HIP events on its own stream.  read waits for the recorded
+ * events, writes up to max (kind, milliseconds) pairs and returns how many there were.
+ * Kinds: 0 init, 1 histogram, 2 onesweep pass, 3 one-workgroup sort, 4 segmented
+ * binning, 5 segmented histogram, 6 segmented onesweep pass, 7 copy. */
+CCCL_C_API void cccl_amd_profile_enable(int on);
+CCCL_C_API int cccl_amd_profile_read(int* kinds, float* ms, int max);
+
+CCCL_C_EXTERN_C_END
