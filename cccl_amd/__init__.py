@@ -1,0 +1,22 @@
+"""MI355X-native (gfx950) device-wide LSD radix sort behind CUB's / CCCL's radix-sort API.
+
+The native library libcccl_amd.so holds the kernels, the host dispatch and the
+cccl_device_radix_sort_* C-ABI; this package is its Python host binding.
+"""
+
+from .radix_sort import (  # noqa: F401
+    DoubleBuffer,
+    SegmentedSorter,
+    SortError,
+    SortOrder,
+    Sorter,
+    block_sort_shapes,
+    config_valid,
+    default_config,
+    default_segmented_config,
+    make_config,
+    make_radix_sort,
+    onesweep_shapes,
+    radix_sort,
+    segmented_radix_sort,
+)

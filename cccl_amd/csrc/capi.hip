@@ -742,7 +742,10 @@ CCCL_C_API int cccl_amd_radix_sort(
   DoubleBufferRef vals{{vals0, vals1}, *selector};
   const hipError_t e =
     cccl_amd::radix_sort(d_temp_storage, *temp_storage_bytes, keys, vals, pr, c, reinterpret_cast<hipStream_t>(stream));
-  *selector = keys.selector;
+  // Without overwrite the sort may swap internal buffers into its DoubleBuffer; relative
+  // to the caller's (keys0, keys1) the result is in keys1 whenever work was done.
+  const bool did_work = d_temp_storage != nullptr && num_items > 0 && !(begin_bit == end_bit && pr.overwrite);
+  *selector           = (pr.overwrite || !did_work) ? keys.selector : 1;
   return int(e);
 }
 

@@ -94,8 +94,7 @@ hipError_t ensure_lds_attribute(const void* fn, size_t lds)
   {
     return hipSuccess;
   }
-  if (hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(kMaxLdsBytes));
-      e != hipSuccess)
+  if (hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)); e != hipSuccess)
   {
     return e;
   }

@@ -1,0 +1,329 @@
+"""Python host API of the MI355X radix sort over the C-ABI.
+
+Mirrors cuda.compute's radix sort
+(python/cuda_cccl/cuda/compute/algorithms/_sort/_radix_sort.py:23-280 and
+_sort_common.py:21-30): ``DoubleBuffer``, ``SortOrder``, ``make_radix_sort`` (a
+reusable sorter built through ``cccl_device_radix_sort_build``), the one-shot
+``radix_sort``, and the two-phase temp-storage protocol (``temp_storage=None``
+returns the byte count).  Device arrays are torch tensors on a ROCm device; the
+stream defaults to torch's current stream.  Every call goes through
+libcccl_amd.so: there is no host fallback.
+
+``segmented_radix_sort`` / ``make_segmented_radix_sort`` expose
+cub::DeviceSegmentedRadixSort (cub/cub/device/device_segmented_radix_sort.cuh:233-270),
+which has no C-ABI in the reference, through the typed entry point
+``cccl_amd_segmented_radix_sort``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import enum
+
+import torch
+
+from . import _bindings as B
+
+
+class SortOrder(enum.Enum):
+    ASCENDING = B.CCCL_ASCENDING
+    DESCENDING = B.CCCL_DESCENDING
+
+
+class DoubleBuffer:
+    """Pair of device arrays; ``selector`` names the current one (cub::DoubleBuffer)."""
+
+    def __init__(self, d_current, d_alternate):
+        self.d_buffers = [d_current, d_alternate]
+        self.selector = 0
+
+    def current(self):
+        return self.d_buffers[self.selector]
+
+    def alternate(self):
+        return self.d_buffers[1 - self.selector]
+
+
+class SortError(RuntimeError):
+    def __init__(self, where: str, code: int):
+        super().__init__(f"{where} failed with code {code}")
+        self.code = code
+
+
+_KEY_TYPES = {
+    torch.int8: B.CCCL_INT8,
+    torch.int16: B.CCCL_INT16,
+    torch.int32: B.CCCL_INT32,
+    torch.int64: B.CCCL_INT64,
+    torch.uint8: B.CCCL_UINT8,
+    torch.uint16: B.CCCL_UINT16,
+    torch.uint32: B.CCCL_UINT32,
+    torch.uint64: B.CCCL_UINT64,
+    torch.float16: B.CCCL_FLOAT16,
+    # bfloat16 orders like any sign-magnitude float of its width: same transform as fp16.
+    torch.bfloat16: B.CCCL_FLOAT16,
+    torch.float32: B.CCCL_FLOAT32,
+    torch.float64: B.CCCL_FLOAT64,
+    torch.bool: B.CCCL_BOOLEAN,
+}
+
+
+def key_type_of(dtype: torch.dtype) -> int:
+    try:
+        return _KEY_TYPES[dtype]
+    except KeyError:
+        raise TypeError(f"unsupported key dtype {dtype}") from None
+
+
+def _ptr(t) -> int | None:
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _item_bytes(t) -> int:
+    return t.element_size() * (t[0].numel() if t.dim() > 1 else 1)
+
+
+def _stream_handle(stream) -> int | None:
+    if stream is None:
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _arrays(d_in_keys, d_out_keys, d_in_values, d_out_values):
+    if isinstance(d_in_keys, DoubleBuffer):
+        ki, ko = d_in_keys.current(), d_in_keys.alternate()
+        if d_in_values is not None:
+            assert isinstance(d_in_values, DoubleBuffer)
+            vi, vo = d_in_values.current(), d_in_values.alternate()
+        else:
+            vi = vo = None
+        return ki, ko, vi, vo
+    return d_in_keys, d_out_keys, d_in_values, d_out_values
+
+
+class _RadixSort:
+    """A built sorter (cccl_device_radix_sort_build result) for one key/value type and order."""
+
+    def __init__(self, d_in_keys, d_out_keys, d_in_values, d_out_values, order: SortOrder):
+        L = B.lib()
+        ki, _, vi, _ = _arrays(d_in_keys, d_out_keys, d_in_values, d_out_values)
+        self.key_dtype = ki.dtype
+        self.key_type = key_type_of(ki.dtype)
+        self.value_bytes = 0 if vi is None else _item_bytes(vi)
+        self.order = order
+        keys_it = B.pointer_iterator(self.key_type, ki.element_size(), None)
+        vals_it = B.pointer_iterator(B.CCCL_STORAGE, self.value_bytes, None)
+        if self.value_bytes:
+            vals_it.state = 1  # any non-null state marks "pairs" at build time
+        self.build = B.cccl_device_radix_sort_build_result_t()
+        cc_major, cc_minor = 9, 5
+        rc = L.cccl_device_radix_sort_build(ctypes.byref(self.build), order.value, keys_it, vals_it, B.identity_op(),
+                                            None, cc_major, cc_minor, None, None, None, None)
+        if rc != B.CUDA_SUCCESS:
+            raise SortError("cccl_device_radix_sort_build", rc)
+
+    def __del__(self):
+        try:
+            if getattr(self, "build", None) is not None and self.build.payload:
+                B.lib().cccl_device_radix_sort_cleanup(ctypes.byref(self.build))
+        except Exception:
+            pass
+
+    def __call__(self, *, temp_storage, d_in_keys, d_out_keys, d_in_values, d_out_values, num_items: int,
+                 begin_bit: int | None = None, end_bit: int | None = None, stream=None) -> int:
+        L = B.lib()
+        ki, ko, vi, vo = _arrays(d_in_keys, d_out_keys, d_in_values, d_out_values)
+        if ki.dtype != self.key_dtype:
+            raise TypeError("key dtype differs from the one the sorter was built for")
+        is_overwrite_okay = isinstance(d_in_keys, DoubleBuffer)
+        if begin_bit is None:
+            begin_bit = 0
+        if end_bit is None:
+            end_bit = ki.element_size() * 8
+        vbytes = self.value_bytes
+        it = lambda t, ty, sz: B.pointer_iterator(ty, sz, _ptr(t))  # noqa: E731
+        k_in, k_out = it(ki, self.key_type, ki.element_size()), it(ko, self.key_type, ki.element_size())
+        v_in, v_out = it(vi, B.CCCL_STORAGE, vbytes), it(vo, B.CCCL_STORAGE, vbytes)
+        if temp_storage is None:
+            d_temp, nbytes = None, ctypes.c_size_t(0)
+        else:
+            d_temp, nbytes = _ptr(temp_storage), ctypes.c_size_t(temp_storage.numel() * temp_storage.element_size())
+        selector = ctypes.c_int(-1)
+        rc = L.cccl_device_radix_sort(self.build, d_temp, ctypes.byref(nbytes), k_in, k_out, v_in, v_out,
+                                      B.identity_op(), num_items, begin_bit, end_bit, is_overwrite_okay,
+                                      ctypes.byref(selector), _stream_handle(stream))
+        if rc != B.CUDA_SUCCESS:
+            raise SortError("cccl_device_radix_sort", rc)
+        if is_overwrite_okay and temp_storage is not None:
+            assert selector.value in (0, 1)
+            d_in_keys.selector = selector.value
+            if d_in_values is not None:
+                d_in_values.selector = selector.value
+        return nbytes.value
+
+
+def make_radix_sort(d_in_keys, d_out_keys, d_in_values, d_out_values, order: SortOrder) -> _RadixSort:
+    return _RadixSort(d_in_keys, d_out_keys, d_in_values, d_out_values, order)
+
+
+def _temp(nbytes: int, like) -> torch.Tensor:
+    return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=like.device)
+
+
+def radix_sort(*, d_in_keys, d_out_keys, d_in_values=None, d_out_values=None, num_items: int, order: SortOrder,
+               begin_bit: int | None = None, end_bit: int | None = None, stream=None):
+    """One-shot sort: query, allocate temp storage, sort."""
+    sorter = make_radix_sort(d_in_keys, d_out_keys, d_in_values, d_out_values, order)
+    kw = dict(d_in_keys=d_in_keys, d_out_keys=d_out_keys, d_in_values=d_in_values, d_out_values=d_out_values,
+              num_items=num_items, begin_bit=begin_bit, end_bit=end_bit, stream=stream)
+    nbytes = sorter(temp_storage=None, **kw)
+    ki = _arrays(d_in_keys, d_out_keys, d_in_values, d_out_values)[0]
+    sorter(temp_storage=_temp(nbytes, ki), **kw)
+
+
+# ---- typed entry points (explicit launch configuration) ---------------------
+
+def default_config(key_bytes: int, value_bytes: int, num_items: int) -> B.radix_sort_config:
+    c = B.radix_sort_config()
+    B.lib().cccl_amd_radix_sort_default_config(key_bytes, value_bytes, num_items, ctypes.byref(c))
+    return c
+
+
+def make_config(onesweep_threads, onesweep_ipt, single_tile_threads=256, single_tile_ipt=16,
+                histogram_blocks_per_cu=4) -> B.radix_sort_config:
+    c = B.radix_sort_config()
+    c.onesweep_threads, c.onesweep_ipt, c.radix_bits = onesweep_threads, onesweep_ipt, 8
+    c.single_tile_threads, c.single_tile_ipt = single_tile_threads, single_tile_ipt
+    c.histogram_blocks_per_cu = histogram_blocks_per_cu
+    return c
+
+
+def config_valid(cfg: B.radix_sort_config, key_bytes: int, value_bytes: int) -> bool:
+    return bool(B.lib().cccl_amd_radix_sort_config_valid(ctypes.byref(cfg), key_bytes, value_bytes))
+
+
+def onesweep_shapes() -> list[tuple[int, int]]:
+    t = (ctypes.c_int * 64)()
+    i = (ctypes.c_int * 64)()
+    n = B.lib().cccl_amd_onesweep_shapes(t, i, 64)
+    return [(t[k], i[k]) for k in range(n)]
+
+
+def block_sort_shapes() -> list[tuple[int, int]]:
+    t = (ctypes.c_int * 64)()
+    i = (ctypes.c_int * 64)()
+    n = B.lib().cccl_amd_block_sort_shapes(t, i, 64)
+    return [(t[k], i[k]) for k in range(n)]
+
+
+class Sorter:
+    """Direct typed sort with a fixed launch configuration and reusable temp storage.
+
+    keys0/keys1 (and vals0/vals1) form a DoubleBuffer; ``overwrite=False`` is the pointer
+    API (keys0 input, keys1 output, inputs untouched).  ``sort`` returns the selector of
+    the buffer holding the result.
+    """
+
+    def __init__(self, key_dtype, value_bytes: int, num_items: int, descending=False, begin_bit=0, end_bit=None,
+                 overwrite=False, config: B.radix_sort_config | None = None, device="cuda"):
+        self.key_type = key_type_of(key_dtype)
+        self.key_bytes = torch.empty(0, dtype=key_dtype).element_size()
+        self.value_bytes = value_bytes
+        self.num_items = num_items
+        self.descending = descending
+        self.begin_bit = begin_bit
+        self.end_bit = self.key_bytes * 8 if end_bit is None else end_bit
+        self.overwrite = overwrite
+        self.config = config
+        self.device = device
+        self.temp = None
+        self.temp_bytes = self._call(None, 0, None, None, None, None, 0, None)[1]
+        self.temp = torch.empty(max(self.temp_bytes, 1), dtype=torch.uint8, device=device)
+
+    def _call(self, temp_ptr, temp_bytes, k0, k1, v0, v1, selector, stream):
+        nbytes = ctypes.c_size_t(temp_bytes)
+        sel = ctypes.c_int(selector)
+        cfg = ctypes.byref(self.config) if self.config is not None else None
+        rc = B.lib().cccl_amd_radix_sort(temp_ptr, ctypes.byref(nbytes), k0, k1, v0, v1, ctypes.byref(sel),
+                                         self.key_type, self.value_bytes, self.num_items, self.begin_bit,
+                                         self.end_bit, int(self.descending), int(self.overwrite), cfg,
+                                         _stream_handle(stream) if temp_ptr is not None else None)
+        if rc != 0:
+            raise SortError("cccl_amd_radix_sort", rc)
+        return sel.value, nbytes.value
+
+    def sort(self, keys0, keys1, vals0=None, vals1=None, selector=0, stream=None) -> int:
+        return self._call(self.temp.data_ptr(), self.temp.numel(), _ptr(keys0), _ptr(keys1), _ptr(vals0),
+                          _ptr(vals1), selector, stream)[0]
+
+
+# ---- segmented ----------------------------------------------------------------
+
+def default_segmented_config(key_bytes: int, value_bytes: int) -> B.segmented_config:
+    c = B.segmented_config()
+    B.lib().cccl_amd_segmented_default_config(key_bytes, value_bytes, ctypes.byref(c))
+    return c
+
+
+class SegmentedSorter:
+    """cub::DeviceSegmentedRadixSort with size-binned batching and reusable temp storage."""
+
+    def __init__(self, key_dtype, value_bytes: int, num_items: int, num_segments: int, offset_bytes=4,
+                 descending=False, begin_bit=0, end_bit=None, overwrite=False,
+                 config: B.segmented_config | None = None, device="cuda"):
+        self.key_type = key_type_of(key_dtype)
+        self.key_bytes = torch.empty(0, dtype=key_dtype).element_size()
+        self.value_bytes = value_bytes
+        self.num_items = num_items
+        self.num_segments = num_segments
+        self.offset_bytes = offset_bytes
+        self.descending = descending
+        self.begin_bit = begin_bit
+        self.end_bit = self.key_bytes * 8 if end_bit is None else end_bit
+        self.overwrite = overwrite
+        self.config = config if config is not None else default_segmented_config(self.key_bytes, value_bytes)
+        self.temp_bytes = self._call(None, 0, None, None, None, None, 0, None, None, None)[1]
+        self.temp = torch.empty(max(self.temp_bytes, 1), dtype=torch.uint8, device=device)
+
+    def _call(self, temp_ptr, temp_bytes, k0, k1, v0, v1, selector, begin, end, stream):
+        nbytes = ctypes.c_size_t(temp_bytes)
+        sel = ctypes.c_int(selector)
+        rc = B.lib().cccl_amd_segmented_radix_sort(
+            temp_ptr, ctypes.byref(nbytes), k0, k1, v0, v1, ctypes.byref(sel), self.key_type, self.value_bytes,
+            self.num_items, self.num_segments, begin, end, self.offset_bytes, self.begin_bit, self.end_bit,
+            int(self.descending), int(self.overwrite), ctypes.byref(self.config),
+            _stream_handle(stream) if temp_ptr is not None else None)
+        if rc != 0:
+            raise SortError("cccl_amd_segmented_radix_sort", rc)
+        return sel.value, nbytes.value
+
+    def sort(self, keys0, keys1, begin_offsets, end_offsets, vals0=None, vals1=None, selector=0, stream=None) -> int:
+        return self._call(self.temp.data_ptr(), self.temp.numel(), _ptr(keys0), _ptr(keys1), _ptr(vals0),
+                          _ptr(vals1), selector, _ptr(begin_offsets), _ptr(end_offsets), stream)[0]
+
+
+def segmented_radix_sort(*, d_in_keys, d_out_keys, d_in_values=None, d_out_values=None, num_items: int,
+                         num_segments: int, start_offsets, end_offsets, order: SortOrder,
+                         begin_bit: int | None = None, end_bit: int | None = None, stream=None):
+    """One-shot segmented sort (pointer or DoubleBuffer form)."""
+    ki, ko, vi, vo = _arrays(d_in_keys, d_out_keys, d_in_values, d_out_values)
+    overwrite = isinstance(d_in_keys, DoubleBuffer)
+    s = SegmentedSorter(ki.dtype, 0 if vi is None else _item_bytes(vi), num_items, num_segments,
+                        offset_bytes=start_offsets.element_size(), descending=order == SortOrder.DESCENDING,
+                        begin_bit=begin_bit or 0, end_bit=end_bit, overwrite=overwrite, device=ki.device)
+    if overwrite:
+        sel = s.sort(d_in_keys.d_buffers[0], d_in_keys.d_buffers[1], start_offsets, end_offsets,
+                     None if d_in_values is None else d_in_values.d_buffers[0],
+                     None if d_in_values is None else d_in_values.d_buffers[1], selector=d_in_keys.selector,
+                     stream=stream)
+        d_in_keys.selector = sel
+        if d_in_values is not None:
+            d_in_values.selector = sel
+    else:
+        s.sort(ki, ko, start_offsets, end_offsets, vi, vo, stream=stream)

@@ -55,7 +55,8 @@ typedef struct cccl_amd_segmented_config_t
  * keys0/keys1 with *selector form a cub::DoubleBuffer; the pointer API is
  * is_overwrite_okay = 0 with keys0 = input and keys1 = output.  vals0 == NULL sorts keys
  * only.  value_bytes must be 1, 2, 4, 8 or 16 (or 0 for keys only).  cfg == NULL picks
- * the built-in default.  On return *selector names the buffer holding the result. */
+ * the built-in default.  On return *selector names the buffer holding the result: with
+ * is_overwrite_okay = 0 that is 1 (keys1) whenever the call sorted or copied anything. */
 CCCL_C_API int cccl_amd_radix_sort(
   void* d_temp_storage,
   size_t* temp_storage_bytes,

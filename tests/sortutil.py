@@ -1,0 +1,101 @@
+"""Input generators and host/device helpers shared by the tests and the benchmark."""
+
+from __future__ import annotations
+
+import numpy as np
+
+ENTROPY_LEVELS = ["uniform", "and2", "and3", "and4", "and5", "equal"]
+
+
+def random_keys(rng: np.random.Generator, dtype, n: int, entropy: str = "uniform") -> np.ndarray:
+    """Keys of `dtype`; entropy reduction by AND of k uniform words (nvbench_helper's
+    entropy generator, nvbench_helper/nvbench_helper/nvbench_helper.cu:344-402)."""
+    dt = np.dtype(dtype)
+    if entropy == "equal":
+        word = uniform_bits(rng, dt, 1)
+        return np.repeat(word, n)
+    if dt.kind == "f" and entropy == "uniform":
+        return rng.standard_normal(n).astype(dt)
+    k = {"uniform": 1, "and2": 2, "and3": 3, "and4": 4, "and5": 5}[entropy]
+    u = np.dtype(f"u{dt.itemsize}")
+    out = uniform_bits(rng, u, n)
+    for _ in range(k - 1):
+        out &= uniform_bits(rng, u, n)
+    return out.view(dt)
+
+
+def uniform_bits(rng: np.random.Generator, dt, n: int) -> np.ndarray:
+    dt = np.dtype(dt)
+    u = np.dtype(f"u{dt.itemsize}")
+    if dt.itemsize == 8:
+        words = rng.integers(0, 2**64, n, dtype=np.uint64, endpoint=False)
+    else:
+        words = rng.integers(0, 2 ** (8 * dt.itemsize), n, dtype=np.uint64).astype(u)
+    return words.astype(u, copy=False).view(dt)
+
+
+def float_keys_with_specials(rng: np.random.Generator, dtype, n: int, zero_frac=0.01, inf_frac=0.001) -> np.ndarray:
+    """N(0,1) floats with ~zero_frac of items set to -0.0/+0.0 and ~inf_frac to +-inf."""
+    k = rng.standard_normal(n).astype(dtype)
+    nz = int(n * zero_frac)
+    ni = int(n * inf_frac)
+    idx = rng.permutation(n)[: nz + ni]
+    z = idx[:nz]
+    k[z[: nz // 2]] = -0.0
+    k[z[nz // 2:]] = 0.0
+    inf = idx[nz:]
+    k[inf[: ni // 2]] = -np.inf
+    k[inf[ni // 2:]] = np.inf
+    return k
+
+
+def lognormal_segments(rng: np.random.Generator, num_segments: int, median: float, sigma: float, lo: int, hi: int,
+                       total: int | None = None) -> np.ndarray:
+    """Segment sizes drawn lognormal(median, sigma), clipped to [lo, hi]; with `total`,
+    rescaled so the sizes sum to `total` (then re-clipped and the remainder spread)."""
+    s = np.exp(np.log(median) + sigma * rng.standard_normal(num_segments))
+    s = np.clip(np.rint(s), lo, hi).astype(np.int64)
+    if total is not None:
+        s = np.clip(np.rint(s * (total / s.sum())), lo, hi).astype(np.int64)
+        diff = total - int(s.sum())
+        i = 0
+        order = rng.permutation(num_segments)
+        while diff != 0 and i < 10 * num_segments:
+            j = order[i % num_segments]
+            step = 1 if diff > 0 else -1
+            room = (hi - s[j]) if step > 0 else (s[j] - lo)
+            take = min(abs(diff), room)
+            if take > 0:
+                s[j] += step * take
+                diff -= step * take
+            i += 1
+    return s
+
+
+def to_dev(a: np.ndarray, dtype=None, device="cuda"):
+    """Copies raw bytes of a host array to the device and views them as `dtype`."""
+    import torch
+
+    a = np.ascontiguousarray(a)
+    t = torch.from_numpy(a.view(np.uint8).reshape(-1)).to(device)
+    if dtype is None and a.dtype.kind == "V":
+        return t.view(a.size, a.dtype.itemsize)  # opaque records: one row per item
+    if dtype is None:
+        dtype = torch_dtype(a.dtype)
+    return t.view(dtype)
+
+
+def to_host(t, np_dtype) -> np.ndarray:
+    return t.contiguous().view(__import__("torch").uint8).cpu().numpy().reshape(-1).view(np_dtype)
+
+
+def torch_dtype(np_dtype):
+    import torch
+
+    m = {
+        np.dtype(np.uint8): torch.uint8, np.dtype(np.int8): torch.int8, np.dtype(np.uint16): torch.uint16,
+        np.dtype(np.int16): torch.int16, np.dtype(np.uint32): torch.uint32, np.dtype(np.int32): torch.int32,
+        np.dtype(np.uint64): torch.uint64, np.dtype(np.int64): torch.int64, np.dtype(np.float16): torch.float16,
+        np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64, np.dtype(np.bool_): torch.bool,
+    }
+    return m[np.dtype(np_dtype)]

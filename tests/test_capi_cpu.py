@@ -1,0 +1,218 @@
+"""C-ABI contract checks that need no GPU: the library loads, exports every symbol the
+headers declare, and the build / compile / load / serialize / deserialize / cleanup and
+temp-storage-query paths behave like c/parallel/src/radix_sort.cu (error codes of
+:465-493, :599-607, :779-783, :816-947)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from cccl_amd import _bindings as B
+
+REPO = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    names = set()
+    for h in (REPO / "include").rglob("*.h"):
+        text = h.read_text()
+        for m in re.finditer(r"CCCL_C_API\s+(?:[\w\s\*]+?)\b(\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = B.lib()
+    decl = declared_symbols()
+    assert len(decl) >= 19
+    assert decl == set(B.EXPORTED_SYMBOLS)
+    for name in decl:
+        assert hasattr(L, name), name
+
+
+def test_reference_entry_points_present():
+    """The nine entry points of c/parallel/include/cccl/c/radix_sort.h:66-145."""
+    for name in ["cccl_device_radix_sort_build", "cccl_device_radix_sort_build_ex", "cccl_device_radix_sort_compile",
+                 "cccl_device_radix_sort_load", "cccl_device_radix_sort", "cccl_device_radix_sort_link_ltoir",
+                 "cccl_device_radix_sort_serialize", "cccl_device_radix_sort_deserialize",
+                 "cccl_device_radix_sort_cleanup"]:
+        assert name in declared_symbols()
+
+
+def _build(key_type=B.CCCL_UINT32, key_size=4, value_size=4, order=B.CCCL_ASCENDING, decomposer=None,
+           cc=(9, 5)):
+    L = B.lib()
+    br = B.cccl_device_radix_sort_build_result_t()
+    kit = B.pointer_iterator(key_type, key_size, None)
+    vit = B.pointer_iterator(B.CCCL_STORAGE, value_size, 1 if value_size else None)
+    op = B.identity_op() if decomposer is None else decomposer
+    rc = L.cccl_device_radix_sort_build(ctypes.byref(br), order, kit, vit, op, None, cc[0], cc[1], None, None, None,
+                                        None)
+    return rc, br
+
+
+def test_build_load_cleanup():
+    L = B.lib()
+    rc, br = _build()
+    assert rc == B.CUDA_SUCCESS
+    assert br.payload and br.payload_size > 0 and br.runtime_policy and br.runtime_policy_size > 0
+    assert br.onesweep_kernel and br.histogram_kernel and br.single_tile_kernel and br.init_bins_and_counters_kernel
+    assert br.key_type.size == 4 and br.value_type.size == 4 and br.order == B.CCCL_ASCENDING
+    assert b"onesweep" in br.onesweep_kernel_lowered_name
+    assert L.cccl_device_radix_sort_cleanup(ctypes.byref(br)) == B.CUDA_SUCCESS
+    assert not br.payload and not br.runtime_policy
+
+
+def test_keys_only_build_has_empty_value_type():
+    rc, br = _build(value_size=0)
+    assert rc == B.CUDA_SUCCESS and br.value_type.size == 0
+    B.lib().cccl_device_radix_sort_cleanup(ctypes.byref(br))
+
+
+def test_build_rejects_custom_decomposer_and_other_arch():
+    op = B.cccl_op_t()
+    op.name = b"my_decomposer"
+    rc, _ = _build(decomposer=op)
+    assert rc == B.CUDA_ERROR_NOT_SUPPORTED
+    rc, _ = _build(cc=(9, 0))
+    assert rc == B.CUDA_ERROR_NOT_SUPPORTED
+    rc, _ = _build(key_type=B.CCCL_STORAGE, key_size=12)
+    assert rc == B.CUDA_ERROR_NOT_SUPPORTED
+    rc, _ = _build(value_size=12)
+    assert rc == B.CUDA_ERROR_NOT_SUPPORTED
+
+
+def test_null_and_incomplete_build_results():
+    L = B.lib()
+    assert L.cccl_device_radix_sort_load(None) == B.CUDA_ERROR_INVALID_VALUE
+    empty = B.cccl_device_radix_sort_build_result_t()
+    assert L.cccl_device_radix_sort_load(ctypes.byref(empty)) == B.CUDA_ERROR_INVALID_VALUE
+    buf, size = ctypes.c_void_p(), ctypes.c_size_t()
+    assert L.cccl_device_radix_sort_serialize(ctypes.byref(empty), ctypes.byref(buf),
+                                              ctypes.byref(size)) == B.CUDA_ERROR_INVALID_VALUE
+    assert L.cccl_device_radix_sort_cleanup(None) == B.CUDA_ERROR_INVALID_VALUE
+
+
+def test_serialize_round_trip_then_load():
+    L = B.lib()
+    rc, br = _build(key_type=B.CCCL_FLOAT32, order=B.CCCL_DESCENDING, value_size=8)
+    assert rc == 0
+    buf, size = ctypes.c_void_p(), ctypes.c_size_t()
+    assert L.cccl_device_radix_sort_serialize(ctypes.byref(br), ctypes.byref(buf), ctypes.byref(size)) == 0
+    assert size.value > 0
+    blob = ctypes.string_at(buf, size.value)
+    br2 = B.cccl_device_radix_sort_build_result_t()
+    assert L.cccl_device_radix_sort_deserialize(ctypes.byref(br2), buf, size) == 0
+    L.cccl_serialization_buffer_free(buf)
+    assert not br2.onesweep_kernel  # handles come from a later load
+    assert br2.order == B.CCCL_DESCENDING and br2.key_type.type == B.CCCL_FLOAT32 and br2.value_type.size == 8
+    assert br2.onesweep_kernel_lowered_name == br.onesweep_kernel_lowered_name
+    assert ctypes.string_at(br2.payload, br2.payload_size) == ctypes.string_at(br.payload, br.payload_size)
+    assert L.cccl_device_radix_sort_load(ctypes.byref(br2)) == 0
+    assert br2.onesweep_kernel
+    # a corrupted blob is rejected: bad order field
+    bad = bytearray(blob)
+    off = 8 + 4 + 4 + 20 + 20
+    bad[off:off + 4] = (7).to_bytes(4, "little")
+    br3 = B.cccl_device_radix_sort_build_result_t()
+    assert L.cccl_device_radix_sort_deserialize(ctypes.byref(br3), bytes(bad), len(bad)) == B.CUDA_ERROR_INVALID_VALUE
+    # truncated blob
+    assert L.cccl_device_radix_sort_deserialize(ctypes.byref(br3), blob[:30], 30) == B.CUDA_ERROR_INVALID_VALUE
+    for b in (br, br2):
+        L.cccl_device_radix_sort_cleanup(ctypes.byref(b))
+
+
+def test_link_ltoir():
+    L = B.lib()
+    rc, br = _build()
+    assert L.cccl_device_radix_sort_link_ltoir(ctypes.byref(br), None, None, 0) == 0
+    assert L.cccl_device_radix_sort_link_ltoir(ctypes.byref(br), None, None, 1) == B.CUDA_ERROR_NOT_SUPPORTED
+    L.cccl_device_radix_sort_cleanup(ctypes.byref(br))
+
+
+def _run(br, d_temp, nbytes, n, begin=0, end=32, overwrite=False, kinds=None, keys=(0x10000, 0x20000),
+         vals=(0x30000, 0x40000)):
+    L = B.lib()
+    kinds = kinds or [B.CCCL_POINTER] * 4
+    its = []
+    for kind, ptr, ty, sz in zip(kinds, [keys[0], keys[1], vals[0], vals[1]],
+                                 [B.CCCL_UINT32, B.CCCL_UINT32, B.CCCL_STORAGE, B.CCCL_STORAGE], [4, 4, 4, 4]):
+        it = B.pointer_iterator(ty, sz, ptr)
+        it.type = kind
+        its.append(it)
+    size = ctypes.c_size_t(nbytes)
+    sel = ctypes.c_int(-1)
+    rc = L.cccl_device_radix_sort(br, d_temp, ctypes.byref(size), *its, B.identity_op(), n, begin, end, overwrite,
+                                  ctypes.byref(sel), None)
+    return rc, size.value, sel.value
+
+
+def test_temp_storage_query_launches_nothing():
+    """The query writes bytes > 0 and touches no device memory (pointers are fake)."""
+    rc, br = _build()
+    for n in (0, 1, 4096, 4097, 1 << 20, 1 << 28):
+        rc, nbytes, _ = _run(br, None, 0, n)
+        assert rc == 0 and nbytes > 0
+    rc, small, _ = _run(br, None, 0, 1 << 20, overwrite=True)
+    rc, big, _ = _run(br, None, 0, 1 << 20, overwrite=False)
+    assert big >= small + (1 << 20) * 8  # pointer form needs the extra key + value buffers
+    B.lib().cccl_device_radix_sort_cleanup(ctypes.byref(br))
+
+
+def test_undersized_temp_storage_is_invalid_value():
+    rc, br = _build()
+    _, need, _ = _run(br, None, 0, 1 << 20)
+    rc, _, _ = _run(br, 0x1000, need - 1, 1 << 20)
+    assert rc == B.CUDA_ERROR_INVALID_VALUE
+    B.lib().cccl_device_radix_sort_cleanup(ctypes.byref(br))
+
+
+def test_non_pointer_iterator_is_rejected():
+    rc, br = _build()
+    rc, _, _ = _run(br, None, 0, 100, kinds=[B.CCCL_POINTER, B.CCCL_ITERATOR, B.CCCL_POINTER, B.CCCL_POINTER])
+    assert rc == B.CUDA_ERROR_UNKNOWN
+    B.lib().cccl_device_radix_sort_cleanup(ctypes.byref(br))
+
+
+def test_trivial_paths_selector():
+    """n == 0 and (begin == end with overwrite) do nothing: selector stays 0, no launch."""
+    rc, br = _build()
+    rc, _, sel = _run(br, 0x1000, 1, 0)
+    assert rc == 0 and sel == 0
+    rc, _, sel = _run(br, 0x1000, 1, 1000, begin=7, end=7, overwrite=True)
+    assert rc == 0 and sel == 0
+    rc, nbytes, _ = _run(br, None, 0, 1000, begin=7, end=7, overwrite=False)
+    assert rc == 0 and nbytes == 1
+    B.lib().cccl_device_radix_sort_cleanup(ctypes.byref(br))
+
+
+def test_invalid_bit_range():
+    rc, br = _build()
+    rc, _, _ = _run(br, None, 0, 100, begin=0, end=33)
+    assert rc == B.CUDA_ERROR_INVALID_VALUE
+    rc, _, _ = _run(br, None, 0, 100, begin=9, end=8)
+    assert rc == B.CUDA_ERROR_INVALID_VALUE
+    B.lib().cccl_device_radix_sort_cleanup(ctypes.byref(br))
+
+
+def test_configs_and_shapes():
+    import cccl_amd as ca
+
+    shapes = ca.onesweep_shapes()
+    assert (512, 16) in shapes and (256, 8) in shapes
+    for t, i in shapes:
+        assert ca.config_valid(ca.make_config(t, i), 4, 4)
+    assert not ca.config_valid(ca.make_config(384, 23), 4, 4)
+    c = ca.default_config(4, 4, 1 << 28)
+    assert ca.config_valid(c, 4, 4)
+    # a tile that does not fit in 160 KiB of LDS is not offered
+    assert not ca.config_valid(ca.make_config(512, 16), 8, 16)
+
+
+def test_segmented_query_only():
+    import cccl_amd as ca
+
+    s = ca.SegmentedSorter(__import__("torch").uint32, 4, 1 << 20, 1000, device="cpu")
+    assert s.temp_bytes > 0

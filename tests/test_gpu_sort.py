@@ -1,0 +1,269 @@
+"""GPU parity of the device-wide sort against the CPU oracle, through the C-ABI.
+
+Mirrors the reference's device radix sort tests
+(cub/test/catch2_test_device_radix_sort_keys.cu:75-519,
+catch2_test_device_radix_sort_pairs.cu:34-202) and the C-ABI tests
+(c/parallel/test/test_radix_sort.cpp:181-329): key types x orders x sizes, bit windows,
++-0 / NaN / inf floats, entropy reduction, all-equal keys, DoubleBuffer selector,
+overwrite and pointer forms.  Results must be bit-identical.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from sortutil import float_keys_with_specials, random_keys, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import cccl_amd  # noqa: F401  (fails loudly if the library is missing)
+
+
+def run_pairs(keys, vals, descending=False, begin_bit=None, end_bit=None, double_buffer=False):
+    import cccl_amd as ca
+
+    kd = to_dev(keys)
+    ko = torch.empty_like(kd)
+    vd = vo = None
+    if vals is not None:
+        vd = to_dev(vals)
+        vo = torch.empty_like(vd)
+    order = ca.SortOrder.DESCENDING if descending else ca.SortOrder.ASCENDING
+    if double_buffer:
+        kb = ca.DoubleBuffer(kd, ko)
+        vb = ca.DoubleBuffer(vd, vo) if vals is not None else None
+        ca.radix_sort(d_in_keys=kb, d_out_keys=None, d_in_values=vb, d_out_values=None, num_items=keys.size,
+                      order=order, begin_bit=begin_bit, end_bit=end_bit)
+        torch.cuda.synchronize()
+        rk = to_host(kb.current(), keys.dtype)
+        rv = to_host(vb.current(), vals.dtype) if vals is not None else None
+        if vals is not None:
+            assert kb.selector == vb.selector
+        return rk, rv, kb.selector
+    ca.radix_sort(d_in_keys=kd, d_out_keys=ko, d_in_values=vd, d_out_values=vo, num_items=keys.size, order=order,
+                  begin_bit=begin_bit, end_bit=end_bit)
+    torch.cuda.synchronize()
+    # pointer form leaves the input untouched
+    assert np.array_equal(to_host(kd, keys.dtype).view(np.uint8), np.ascontiguousarray(keys).view(np.uint8))
+    return to_host(ko, keys.dtype), (to_host(vo, vals.dtype) if vals is not None else None), None
+
+
+def assert_same(got_k, got_v, exp_k, exp_v):
+    u = f"u{exp_k.dtype.itemsize}"
+    assert np.array_equal(got_k.view(u), exp_k.view(u)), "keys differ"
+    if exp_v is not None:
+        assert np.array_equal(got_v.view(np.uint8), exp_v.view(np.uint8)), "values differ"
+
+
+SIZES = [1, 7, 64, 1000, 4096, 4097, 65536, 100003, 1 << 20]
+KEY_TYPES = [np.uint8, np.int8, np.uint16, np.int16, np.uint32, np.int32, np.uint64, np.int64, np.float16,
+             np.float32, np.float64]
+
+
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("descending", [False, True])
+def test_u32_pairs(n, descending):
+    rng = np.random.default_rng(n)
+    keys = random_keys(rng, np.uint32, n)
+    vals = np.arange(n, dtype=np.uint32)
+    gk, gv, _ = run_pairs(keys, vals, descending)
+    ek, ev = oracle.sort(keys, vals, descending=descending)
+    assert_same(gk, gv, ek, ev)
+
+
+@pytest.mark.parametrize("dtype", KEY_TYPES)
+@pytest.mark.parametrize("descending", [False, True])
+@pytest.mark.parametrize("n", [3000, 250001])
+def test_key_types_keys_only(dtype, descending, n):
+    rng = np.random.default_rng(17 + n)
+    if np.dtype(dtype).kind == "f":
+        keys = float_keys_with_specials(rng, dtype, n, zero_frac=0.05, inf_frac=0.01)
+    else:
+        keys = random_keys(rng, dtype, n)
+    gk, _, _ = run_pairs(keys, None, descending)
+    ek, _ = oracle.sort(keys, None, descending=descending)
+    assert_same(gk, None, ek, None)
+
+
+@pytest.mark.parametrize("vdtype", [np.uint8, np.uint16, np.uint64, "V16"])
+def test_value_widths(vdtype):
+    rng = np.random.default_rng(99)
+    n = 123457
+    keys = random_keys(rng, np.uint32, n, "and2")
+    if vdtype == "V16":
+        vals = rng.integers(0, 255, n * 16, dtype=np.uint8).view("V16")
+    else:
+        vals = rng.integers(0, np.iinfo(vdtype).max, n, dtype=vdtype)
+    for desc in (False, True):
+        gk, gv, _ = run_pairs(keys, vals, desc)
+        ek, ev = oracle.sort(keys, vals, descending=desc)
+        assert_same(gk, gv, ek, ev)
+
+
+@pytest.mark.parametrize("entropy", ["uniform", "and2", "and3", "and4", "and5", "equal"])
+@pytest.mark.parametrize("n", [5000, 1 << 21])
+def test_entropy_levels_stable(entropy, n):
+    rng = np.random.default_rng(1234)
+    keys = random_keys(rng, np.uint32, n, entropy)
+    vals = np.arange(n, dtype=np.uint32)
+    gk, gv, _ = run_pairs(keys, vals)
+    ek, ev = oracle.sort(keys, vals)
+    assert_same(gk, gv, ek, ev)
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.int64, np.float32, np.uint16])
+@pytest.mark.parametrize("frac", [(0, 1 / 3), (1 / 3, 3 / 4), (3 / 4, 1.0), (0, 0.0), (0.5, 0.5)])
+def test_bit_windows(dtype, frac):
+    """catch2_test_device_radix_sort_keys.cu:115-161: begin/end in {0, b/3, 3b/4, b}."""
+    rng = np.random.default_rng(5)
+    bits = np.dtype(dtype).itemsize * 8
+    b, e = int(bits * frac[0]), int(bits * frac[1])
+    n = 70001
+    keys = random_keys(rng, dtype, n)
+    vals = np.arange(n, dtype=np.uint32)
+    for desc in (False, True):
+        for db in (False, True):
+            gk, gv, _ = run_pairs(keys, vals, desc, b, e, double_buffer=db)
+            ek, ev = oracle.sort(keys, vals, descending=desc, begin_bit=b, end_bit=e)
+            assert_same(gk, gv, ek, ev)
+
+
+def test_u64_window_0_40_descending():
+    rng = np.random.default_rng(40)
+    n = 1 << 20
+    keys = random_keys(rng, np.uint64, n)
+    vals = np.arange(n, dtype=np.int32)
+    gk, gv, sel = run_pairs(keys, vals, True, 0, 40, double_buffer=True)
+    ek, ev = oracle.sort(keys, vals, descending=True, begin_bit=0, end_bit=40)
+    assert_same(gk, gv, ek, ev)
+    assert sel == 1  # five passes flip the DoubleBuffer selector
+
+
+@pytest.mark.parametrize("descending", [False, True])
+def test_float_pm_zero_bitwise(descending):
+    """catch2_test_device_radix_sort_keys.cu:165-219: +-0 kept bitwise, in input order."""
+    rng = np.random.default_rng(0)
+    n = 200000
+    keys = float_keys_with_specials(rng, np.float32, n, zero_frac=0.3, inf_frac=0.05)
+    vals = np.arange(n, dtype=np.int32)
+    gk, gv, _ = run_pairs(keys, vals, descending)
+    ek, ev = oracle.sort(keys, vals, descending=descending)
+    assert_same(gk, gv, ek, ev)
+
+
+def test_float_nan_bit_pattern():
+    """catch2_test_device_radix_sort_keys.cu:221-283: NaNs sort by bit pattern."""
+    rng = np.random.default_rng(1)
+    n = 50000
+    keys = rng.standard_normal(n).astype(np.float32)
+    idx = rng.permutation(n)[:500]
+    nan_bits = (rng.integers(0, 2**22, 500).astype(np.uint32) | np.uint32(0x7FC00000))
+    nan_bits[::2] |= np.uint32(0x80000000)
+    keys.view(np.uint32)[idx] = nan_bits
+    for desc in (False, True):
+        gk, _, _ = run_pairs(keys, None, desc)
+        ek, _ = oracle.sort(keys, None, descending=desc)
+        assert_same(gk, None, ek, None)
+
+
+@pytest.mark.parametrize("n", [1, 100, 4096, 5000, 1 << 20])
+@pytest.mark.parametrize("end_bit", [32, 24, 16, 8])
+def test_double_buffer_selector(n, end_bit):
+    """catch2_test_device_radix_sort_keys.cu:418-447: result in d_keys.Current()."""
+    rng = np.random.default_rng(n + end_bit)
+    keys = random_keys(rng, np.uint32, n)
+    vals = np.arange(n, dtype=np.uint32)
+    gk, gv, sel = run_pairs(keys, vals, False, 0, end_bit, double_buffer=True)
+    ek, ev = oracle.sort(keys, vals, begin_bit=0, end_bit=end_bit)
+    assert_same(gk, gv, ek, ev)
+    passes = (end_bit + 7) // 8
+    single_tile = n <= 4096
+    assert sel == (1 if single_tile else passes & 1)
+
+
+def test_empty_and_no_bits():
+    import cccl_amd as ca
+
+    keys = np.arange(10, dtype=np.uint32)[::-1].copy()
+    # n == 0: nothing happens
+    kd = to_dev(keys)
+    ko = torch.full_like(kd.view(torch.int32), -1)
+    ca.radix_sort(d_in_keys=kd, d_out_keys=ko.view(torch.uint32), num_items=0, order=ca.SortOrder.ASCENDING)
+    torch.cuda.synchronize()
+    assert (ko == -1).all()
+    # begin == end, pointer form: copy
+    ca.radix_sort(d_in_keys=kd, d_out_keys=ko.view(torch.uint32), num_items=10, order=ca.SortOrder.ASCENDING,
+                  begin_bit=5, end_bit=5)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(ko, np.uint32), keys)
+    # begin == end, DoubleBuffer: no-op, selector unchanged
+    db = ca.DoubleBuffer(kd, ko.view(torch.uint32))
+    ca.radix_sort(d_in_keys=db, d_out_keys=None, num_items=10, order=ca.SortOrder.ASCENDING, begin_bit=3,
+                  end_bit=3)
+    assert db.selector == 0
+
+
+@pytest.mark.parametrize("shape", [(256, 8), (256, 16), (512, 8), (512, 12), (512, 16)])
+@pytest.mark.parametrize("n", [8191, 300000])
+def test_every_onesweep_shape(shape, n):
+    import cccl_amd as ca
+
+    rng = np.random.default_rng(shape[0] * shape[1] + n)
+    keys = random_keys(rng, np.uint32, n, "and2")
+    vals = np.arange(n, dtype=np.uint32)
+    cfg = ca.make_config(shape[0], shape[1], single_tile_threads=0)
+    s = ca.Sorter(torch.uint32, 4, n, config=cfg)
+    kd, vd = to_dev(keys), to_dev(vals)
+    ko, vo = torch.empty_like(kd), torch.empty_like(vd)
+    sel = s.sort(kd, ko, vd, vo)
+    torch.cuda.synchronize()
+    assert sel == 1  # pointer form: the result is always in the second buffer
+    ek, ev = oracle.sort(keys, vals)
+    assert_same(to_host(ko, np.uint32), to_host(vo, np.uint32), ek, ev)
+
+
+@pytest.mark.parametrize("shape", [(256, 4), (256, 8), (256, 16), (512, 16)])
+def test_every_single_tile_shape(shape):
+    import cccl_amd as ca
+
+    tile = shape[0] * shape[1]
+    for n in (1, tile // 3, tile):
+        rng = np.random.default_rng(n)
+        keys = random_keys(rng, np.int32, n)
+        vals = np.arange(n, dtype=np.uint64)
+        cfg = ca.make_config(256, 8, single_tile_threads=shape[0], single_tile_ipt=shape[1])
+        for desc in (False, True):
+            s = ca.Sorter(torch.int32, 8, n, descending=desc, config=cfg)
+            kd, vd = to_dev(keys), to_dev(vals)
+            ko, vo = torch.empty_like(kd), torch.empty_like(vd)
+            s.sort(kd, ko, vd, vo)
+            torch.cuda.synchronize()
+            ek, ev = oracle.sort(keys, vals, descending=desc)
+            assert_same(to_host(ko, np.int32), to_host(vo, np.uint64), ek, ev)
+
+
+def test_large_sort_properties():
+    """2^26 u32 pairs: sorted, stable, a permutation, keys match their values' inputs."""
+    import cccl_amd as ca
+
+    n = 1 << 26
+    g = torch.Generator(device="cuda").manual_seed(42)
+    kd = torch.randint(-(2**31), 2**31, (n,), dtype=torch.int32, device="cuda", generator=g)
+    vd = torch.arange(n, dtype=torch.int32, device="cuda")
+    ko, vo = torch.empty_like(kd), torch.empty_like(vd)
+    ca.radix_sort(d_in_keys=kd.view(torch.uint32), d_out_keys=ko.view(torch.uint32), d_in_values=vd,
+                  d_out_values=vo, num_items=n, order=ca.SortOrder.ASCENDING)
+    torch.cuda.synchronize()
+    k = ko.to(torch.int64) & 0xFFFFFFFF
+    assert bool((k[1:] >= k[:-1]).all())
+    eq = k[1:] == k[:-1]
+    assert bool((vo[1:][eq] > vo[:-1][eq]).all())
+    assert bool((torch.sort(vo.to(torch.int64)).values == torch.arange(n, device="cuda")).all())
+    assert bool((kd[vo.to(torch.int64)] == ko).all())
