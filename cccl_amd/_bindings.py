@@ -123,7 +123,8 @@ class segmented_config(ctypes.Structure):
         ("block_ipt", ctypes.c_int32 * 4),
         ("large_threads", ctypes.c_int32),
         ("large_ipt", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("strategy", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

@@ -66,6 +66,20 @@ KernelEntry block_sort_entry()
   }
 }
 
+template <class UK, class V, int T, int I>
+KernelEntry seg_pass_entry()
+{
+  using L = SegPassLayout<UK, V, T, I>;
+  if constexpr (L::kBytes <= kMaxLdsBytes)
+  {
+    return {int(sizeof(UK)), value_bytes<V>(), T, I, reinterpret_cast<const void*>(&seg_pass_kernel<UK, V, T, I>), L::kBytes};
+  }
+  else
+  {
+    return {int(sizeof(UK)), value_bytes<V>(), T, I, nullptr, L::kBytes};
+  }
+}
+
 #define CCCL_AMD_ALL_VALUES(MAKE, UK, T, I)                                                          \
   MAKE<UK, NoValue, T, I>(), MAKE<UK, uint8_t, T, I>(), MAKE<UK, uint16_t, T, I>(),                  \
     MAKE<UK, uint32_t, T, I>(), MAKE<UK, uint64_t, T, I>(), MAKE<UK, Value16, T, I>(),
@@ -88,6 +102,11 @@ KeyWidthTable make_table()
     CCCL_AMD_SEG_ONESWEEP_SHAPES(CCCL_AMD_X)
 #undef CCCL_AMD_X
   };
+  static const KernelEntry seg_pass[] = {
+#define CCCL_AMD_X(T, I) CCCL_AMD_ALL_VALUES(seg_pass_entry, UK, T, I)
+    CCCL_AMD_SEG_ONESWEEP_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
   return {reinterpret_cast<const void*>(&histogram_kernel<UK>),
           onesweep,
           sizeof(onesweep) / sizeof(onesweep[0]),
@@ -95,7 +114,9 @@ KeyWidthTable make_table()
           sizeof(block_sort) / sizeof(block_sort[0]),
           reinterpret_cast<const void*>(&seg_histogram_kernel<UK>),
           seg_onesweep,
-          sizeof(seg_onesweep) / sizeof(seg_onesweep[0])};
+          sizeof(seg_onesweep) / sizeof(seg_onesweep[0]),
+          seg_pass,
+          sizeof(seg_pass) / sizeof(seg_pass[0])};
 }
 
 } // namespace inst

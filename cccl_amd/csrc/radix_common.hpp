@@ -84,22 +84,24 @@ __device__ __forceinline__ uint32_t mask_prefix_count(uint64_t m)
 }
 
 // Set of lanes of the wave whose 8-bit digit equals this lane's: one ballot per bit;
-// a lane keeps the lanes that agree with it on that bit.  Lanes whose bit is clear
-// keep the complement of the ballot, lanes whose bit is set keep the ballot itself,
-// written as m ^ (bit - 1) so the select is an xor with a per-lane 0 / ~0 value.
+// a lane keeps the lanes that agree with it on that bit.  With s = this lane's bit
+// sign-extended to 0 / ~0, the lanes agreeing on the bit are ~(ballot ^ s), so each
+// bit costs one bit-field extract, one compare and an xnor + and per 32-lane half.
 template <int BITS>
 __device__ __forceinline__ uint64_t wave_match_digit(uint32_t d)
 {
-  uint64_t peers = __builtin_amdgcn_read_exec();
+  const uint64_t exec = __builtin_amdgcn_read_exec();
+  uint32_t lo         = uint32_t(exec);
+  uint32_t hi         = uint32_t(exec >> 32);
 #pragma unroll
   for (int b = 0; b < BITS; ++b)
   {
-    const uint32_t bit    = (d >> b) & 1u;
-    const uint64_t m      = __builtin_amdgcn_ballot_w64(bit != 0u);
-    const uint64_t select = uint64_t(int64_t(int32_t(bit) - 1)); // bit ? 0 : ~0
-    peers &= m ^ select;
+    const uint32_t s = uint32_t(int32_t(d << (31 - b)) >> 31); // bit b of d as 0 / ~0
+    const uint64_t m = __builtin_amdgcn_ballot_w64(s != 0u);
+    lo &= ~(uint32_t(m) ^ s);
+    hi &= ~(uint32_t(m >> 32) ^ s);
   }
-  return peers;
+  return (uint64_t(hi) << 32) | lo;
 }
 
 // Inclusive scan across the 64 lanes of a wave.

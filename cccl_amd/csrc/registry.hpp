@@ -34,6 +34,8 @@ struct KeyWidthTable
   const void* seg_histogram;
   const KernelEntry* seg_onesweep;
   size_t num_seg_onesweep;
+  const KernelEntry* seg_pass; // one workgroup per segment and pass
+  size_t num_seg_pass;
 };
 
 const KeyWidthTable& kernels_k1();

@@ -29,6 +29,102 @@ constexpr T ceil_div(T a, T b)
   return (a + b - 1) / b;
 }
 
+// CUB's schedule (dispatch_segmented_radix_sort.cuh:678-893): one launch per digit
+// pass, one workgroup per segment.  Buffer rules as the batched path.
+hipError_t segmented_per_segment_passes(
+  void* d_temp_storage,
+  size_t& temp_storage_bytes,
+  DoubleBufferRef& keys,
+  DoubleBufferRef& vals,
+  const SortProblem& pr,
+  int64_t num_segments,
+  const void* d_begin_offsets,
+  const void* d_end_offsets,
+  int offset_bytes,
+  const cccl_amd_segmented_config_t& cfg,
+  hipStream_t stream)
+{
+  const int kb             = pr.key_bytes;
+  const int vb             = pr.val_bytes;
+  const bool keys_only     = vb == 0;
+  const uint64_t n         = pr.num_items;
+  const KeyWidthTable* tab = kernels_for_key_bytes(kb);
+  const KernelEntry* sp    = find_entry(tab->seg_pass, tab->num_seg_pass, kb, vb, cfg.large_threads, cfg.large_ipt);
+  if (sp == nullptr)
+  {
+    return hipErrorInvalidValue;
+  }
+  const int passes = std::max(ceil_div(pr.end_bit - pr.begin_bit, kRadixBits), 1);
+  size_t sizes[1]  = {pr.overwrite ? 0 : size_t(n) * (kb + vb)};
+  void* parts[1]   = {};
+  if (hipError_t e = alias_temporaries(d_temp_storage, temp_storage_bytes, parts, sizes, 1); e != hipSuccess)
+  {
+    return e;
+  }
+  if (d_temp_storage == nullptr)
+  {
+    return hipSuccess;
+  }
+  char* d_tmp_keys = static_cast<char*>(parts[0]);
+  char* d_tmp_vals = d_tmp_keys + size_t(n) * kb;
+  uint64_t xor_mask = 0;
+  int float_mode    = 0;
+  twiddle_constants(kb, pr.key_kind, pr.descending, xor_mask, float_mode);
+
+  void* key_seq[2];
+  void* val_seq[2];
+  if (pr.overwrite)
+  {
+    key_seq[0] = keys.alternate();
+    key_seq[1] = keys.current();
+    val_seq[0] = keys_only ? nullptr : vals.alternate();
+    val_seq[1] = keys_only ? nullptr : vals.current();
+  }
+  else
+  {
+    void* kf   = keys.alternate();
+    void* vf   = keys_only ? nullptr : vals.alternate();
+    void* vt   = keys_only ? nullptr : static_cast<void*>(d_tmp_vals);
+    key_seq[0] = (passes & 1) ? kf : static_cast<void*>(d_tmp_keys);
+    key_seq[1] = (passes & 1) ? static_cast<void*>(d_tmp_keys) : kf;
+    val_seq[0] = (passes & 1) ? vf : vt;
+    val_seq[1] = (passes & 1) ? vt : vf;
+  }
+  for (int pass = 0; pass < passes; ++pass)
+  {
+    const int bit = pr.begin_bit + pass * kRadixBits;
+    SegPassParams sp_params{};
+    sp_params.keys_in       = pass == 0 ? keys.current() : key_seq[(pass - 1) & 1];
+    sp_params.keys_out      = key_seq[pass & 1];
+    sp_params.vals_in       = keys_only ? nullptr : (pass == 0 ? vals.current() : val_seq[(pass - 1) & 1]);
+    sp_params.vals_out      = val_seq[pass & 1];
+    sp_params.begin_offsets = d_begin_offsets;
+    sp_params.end_offsets   = d_end_offsets;
+    sp_params.num_segments  = num_segments;
+    sp_params.xor_mask      = xor_mask;
+    sp_params.offset_bytes  = offset_bytes;
+    sp_params.float_mode    = float_mode;
+    sp_params.shift         = bit;
+    sp_params.num_bits      = std::max(0, std::min(pr.end_bit - bit, kRadixBits));
+    if (hipError_t e = launch_kernel(sp->fn, uint64_t(num_segments), sp->threads, sp->lds_bytes, stream, &sp_params, 6);
+        e != hipSuccess)
+    {
+      return e;
+    }
+  }
+  if (pr.overwrite)
+  {
+    keys.selector = (keys.selector + passes) & 1;
+    vals.selector = (vals.selector + passes) & 1;
+  }
+  else
+  {
+    keys.selector = (keys.selector + 1) & 1;
+    vals.selector = (vals.selector + 1) & 1;
+  }
+  return hipSuccess;
+}
+
 } // namespace
 
 hipError_t segmented_radix_sort(
@@ -73,6 +169,16 @@ hipError_t segmented_radix_sort(
     return hipSuccess;
   }
   if (num_segments > int64_t(0x7fffffff))
+  {
+    return hipErrorInvalidValue;
+  }
+  if (cfg.strategy == 1)
+  {
+    return segmented_per_segment_passes(
+      d_temp_storage, temp_storage_bytes, keys, vals, pr, num_segments, d_begin_offsets, d_end_offsets, offset_bytes, cfg,
+      stream);
+  }
+  if (cfg.strategy != 0)
   {
     return hipErrorInvalidValue;
   }

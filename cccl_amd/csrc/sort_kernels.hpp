@@ -627,6 +627,197 @@ __global__ void __launch_bounds__(THREADS) block_sort_kernel(BlockSortParams p)
 }
 
 // ---------------------------------------------------------------------------
+// One workgroup per segment and digit pass, looping over the segment's tiles: the
+// schedule of cub::DeviceSegmentedRadixSort (kernel_segmented_radix_sort.cuh:100-291:
+// digit counts over the whole segment, a block scan, then a tile-by-tile stable
+// scatter).  Kept as the reference-equivalent segmented schedule; the size-binned batch
+// is the default.
+// ---------------------------------------------------------------------------
+
+struct SegPassParams
+{
+  const void* keys_in;
+  void* keys_out;
+  const void* vals_in;
+  void* vals_out;
+  const void* begin_offsets;
+  const void* end_offsets;
+  int64_t num_segments;
+  uint64_t xor_mask;
+  int offset_bytes;
+  int float_mode;
+  int shift;
+  int num_bits;
+};
+
+template <class UK, class V, int THREADS, int IPT>
+struct SegPassLayout
+{
+  using Base                       = OnesweepLayout<UK, V, THREADS, IPT>;
+  static constexpr size_t kOffRun  = Base::kBytes + ((16 - Base::kBytes % 16) % 16);
+  static constexpr size_t kBytes   = kOffRun + size_t(kRadix) * 8;
+};
+
+template <class UK, class V, int THREADS, int IPT>
+__global__ void __launch_bounds__(THREADS) seg_pass_kernel(SegPassParams p)
+{
+  using L              = OnesweepLayout<UK, V, THREADS, IPT>;
+  constexpr int kWaves = L::kWaves;
+  constexpr int kTile  = L::kTile;
+  constexpr bool kHasV = L::kHasV;
+  static_assert(THREADS >= kRadix, "one thread per digit is required");
+  using VS = typename std::conditional<kHasV, V, uint8_t>::type;
+
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t* s_wcnt = reinterpret_cast<uint32_t*>(smem);
+  uint64_t* s_base = reinterpret_cast<uint64_t*>(smem + L::kOffBase); // tile output base per digit
+  uint32_t* s_misc = reinterpret_cast<uint32_t*>(smem + L::kOffMisc);
+  UK* s_keys       = reinterpret_cast<UK*>(smem + L::kOffKeys);
+  VS* s_vals       = reinterpret_cast<VS*>(smem + L::kOffVals);
+  uint64_t* s_run  = reinterpret_cast<uint64_t*>(smem + SegPassLayout<UK, V, THREADS, IPT>::kOffRun);
+
+  const uint32_t tid   = threadIdx.x;
+  const uint32_t lane  = tid & 63u;
+  const uint32_t wave  = tid >> 6;
+  const uint32_t strip = wave * (IPT * 64) + lane;
+  const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
+  const uint32_t dmask = p.num_bits > 0 ? (1u << p.num_bits) - 1u : 0u;
+  const UK* kin_all    = static_cast<const UK*>(p.keys_in);
+  const VS* vin_all    = static_cast<const VS*>(p.vals_in);
+  UK* kout             = static_cast<UK*>(p.keys_out);
+  VS* vout             = static_cast<VS*>(p.vals_out);
+
+  for (int64_t seg = blockIdx.x; seg < p.num_segments; seg += gridDim.x)
+  {
+    int64_t b, e;
+    if (p.offset_bytes == 4)
+    {
+      b = static_cast<const int32_t*>(p.begin_offsets)[seg];
+      e = static_cast<const int32_t*>(p.end_offsets)[seg];
+    }
+    else
+    {
+      b = static_cast<const int64_t*>(p.begin_offsets)[seg];
+      e = static_cast<const int64_t*>(p.end_offsets)[seg];
+    }
+    if (e <= b)
+    {
+      continue; // block-uniform
+    }
+
+    // Digit counts of the whole segment.
+    for (uint32_t i = tid; i < uint32_t(kRadix); i += THREADS)
+    {
+      s_wcnt[i] = 0;
+    }
+    __syncthreads();
+    for (int64_t i = b + tid; i < e; i += THREADS)
+    {
+      const uint32_t d = extract_digit(tw(kin_all[i]), p.shift, dmask);
+      __hip_atomic_fetch_add(&s_wcnt[d], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    {
+      uint32_t all        = 0;
+      const uint32_t c    = tid < uint32_t(kRadix) ? s_wcnt[tid] : 0u;
+      const uint32_t excl = radix_exclusive_scan<uint32_t>(c, s_misc + 4, all);
+      if (tid < uint32_t(kRadix))
+      {
+        s_run[tid] = uint64_t(b) + excl;
+      }
+    }
+    __syncthreads();
+
+    // Tiles in order: stable rank within the tile, stage in LDS, write each digit's run
+    // after the runs of earlier tiles.
+    for (int64_t t0 = b; t0 < e; t0 += kTile)
+    {
+      const uint32_t valid = uint32_t(min<int64_t>(kTile, e - t0));
+      for (uint32_t i = tid; i < uint32_t(kWaves * kRadix); i += THREADS)
+      {
+        s_wcnt[i] = 0;
+      }
+      UK keys[IPT];
+      VS vals[IPT];
+      uint32_t dig[IPT];
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        const uint32_t j = strip + uint32_t(i * 64);
+        const bool ok    = j < valid;
+        keys[i]          = ok ? kin_all[t0 + j] : UK(0);
+        if constexpr (kHasV)
+        {
+          if (ok)
+          {
+            vals[i] = vin_all[t0 + j];
+          }
+        }
+        dig[i] = ok ? extract_digit(tw(keys[i]), p.shift, dmask) : uint32_t(kRadix - 1);
+      }
+      __syncthreads();
+      uint32_t rank[IPT];
+      wave_rank<IPT>(dig, rank, s_wcnt + wave * kRadix);
+      __syncthreads();
+      uint32_t total = 0;
+      if (tid < uint32_t(kRadix))
+      {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w)
+        {
+          const uint32_t c         = s_wcnt[w * kRadix + tid];
+          s_wcnt[w * kRadix + tid] = total;
+          total += c;
+        }
+      }
+      uint32_t all         = 0;
+      const uint32_t start = radix_exclusive_scan<uint32_t>(total, s_misc + 4, all);
+      if (tid < uint32_t(kRadix))
+      {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w)
+        {
+          s_wcnt[w * kRadix + tid] += start;
+        }
+        const uint32_t valid_total = (tid == uint32_t(kRadix - 1)) ? total - (uint32_t(kTile) - valid) : total;
+        s_base[tid]                = s_run[tid] - start;
+        s_run[tid] += valid_total;
+      }
+      __syncthreads();
+      const uint32_t* wc = s_wcnt + wave * kRadix;
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        const uint32_t pos = wc[dig[i]] + rank[i];
+        s_keys[pos]        = keys[i];
+        if constexpr (kHasV)
+        {
+          s_vals[pos] = vals[i];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        const uint32_t pos = uint32_t(i) * THREADS + tid;
+        if (pos < valid)
+        {
+          const UK k         = s_keys[pos];
+          const uint32_t d   = extract_digit(tw(k), p.shift, dmask);
+          const uint64_t dst = s_base[d] + pos;
+          kout[dst]          = k;
+          if constexpr (kHasV)
+          {
+            vout[dst] = s_vals[pos];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Batched segmented sort: size binning and per-segment histograms.
 // ---------------------------------------------------------------------------
 

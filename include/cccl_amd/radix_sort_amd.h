@@ -38,7 +38,10 @@ typedef struct cccl_amd_radix_sort_config_t
  * segments of more than block_threads[c-1]*block_ipt[c-1] and at most
  * block_threads[c]*block_ipt[c] items, each sorted whole by one workgroup; larger
  * segments are sorted together by one batched onesweep launch per digit pass with
- * tiles of large_threads*large_ipt items. */
+ * tiles of large_threads*large_ipt items.
+ * strategy 1 instead runs CUB's DeviceSegmentedRadixSort schedule: one launch per digit
+ * pass, one workgroup per segment looping over its tiles of large_threads*large_ipt
+ * items (the block classes are ignored). */
 typedef struct cccl_amd_segmented_config_t
 {
   int32_t num_block_classes; /* 0..4 */
@@ -46,7 +49,8 @@ typedef struct cccl_amd_segmented_config_t
   int32_t block_ipt[4];
   int32_t large_threads;
   int32_t large_ipt;
-  int32_t reserved[2];
+  int32_t strategy; /* 0 size-binned batch, 1 one workgroup per segment per pass */
+  int32_t reserved;
 } cccl_amd_segmented_config_t;
 
 /* Device-wide stable LSD sort.  Replaces cub::DeviceRadixSort::Sort{Keys,Pairs}

@@ -1,0 +1,219 @@
+"""GPU parity of the batched segmented sort against the CPU oracle.
+
+Mirrors cub/test/catch2_test_device_segmented_radix_sort_keys.cu:69-552 and _pairs.cu:
+random segment counts, empty segments, non-contiguous ("unspecified") ranges, bit
+windows, both orders, int32/int64 offsets, pointer and DoubleBuffer forms, plus every
+size-class routing (all one-workgroup, all batched-onesweep, mixed).
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from sortutil import lognormal_segments, random_keys, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def contiguous_offsets(sizes):
+    ends = np.cumsum(sizes)
+    return ends - sizes, ends
+
+
+def run_segmented(keys, vals, seg_begin, seg_end, descending=False, begin_bit=0, end_bit=None, overwrite=False,
+                  offset_dtype=np.int32, config=None, fill=0x5A):
+    import cccl_amd as ca
+
+    n = keys.size
+    kd = to_dev(keys)
+    ko = to_dev(np.full(keys.nbytes, fill, np.uint8)).view(kd.dtype)
+    vd = vo = None
+    if vals is not None:
+        vd = to_dev(vals)
+        vo = to_dev(np.full(vals.nbytes, fill + 1, np.uint8)).view(vd.dtype)
+    sb = to_dev(seg_begin.astype(offset_dtype))
+    se = to_dev(seg_end.astype(offset_dtype))
+    s = ca.SegmentedSorter(kd.dtype, 0 if vals is None else vals.dtype.itemsize, n, seg_begin.size,
+                           offset_bytes=np.dtype(offset_dtype).itemsize, descending=descending,
+                           begin_bit=begin_bit, end_bit=end_bit, overwrite=overwrite, config=config)
+    sel = s.sort(kd, ko, sb, se, vd, vo)
+    torch.cuda.synchronize()
+    bufs_k = [to_host(kd, keys.dtype), to_host(ko, keys.dtype)]
+    bufs_v = [to_host(vd, vals.dtype), to_host(vo, vals.dtype)] if vals is not None else [None, None]
+    return bufs_k, bufs_v, sel
+
+
+def expected(keys, vals, seg_begin, seg_end, init_k, init_v, descending, begin_bit, end_bit):
+    return oracle.segmented_sort(keys, vals, seg_begin, seg_end, init_k, init_v, descending=descending,
+                                 begin_bit=begin_bit, end_bit=end_bit)
+
+
+def check(keys, vals, sb, se, descending=False, begin_bit=0, end_bit=None, overwrite=False, offset_dtype=np.int32,
+          config=None):
+    bk, bv, sel = run_segmented(keys, vals, sb, se, descending, begin_bit, end_bit, overwrite, offset_dtype, config)
+    bits = keys.itemsize * 8 if end_bit is None else end_bit
+    passes = max((bits - begin_bit + 7) // 8, 1)
+    if overwrite:
+        assert sel == passes & 1
+        init_k = keys if sel == 0 else np.full(keys.nbytes, 0x5A, np.uint8).view(keys.dtype)
+        init_v = None
+        if vals is not None:
+            init_v = vals if sel == 0 else np.full(vals.nbytes, 0x5B, np.uint8).view(vals.dtype)
+    else:
+        assert sel == 1
+        assert np.array_equal(bk[0].view(np.uint8), keys.view(np.uint8))  # input untouched
+        init_k = np.full(keys.nbytes, 0x5A, np.uint8).view(keys.dtype)
+        init_v = None if vals is None else np.full(vals.nbytes, 0x5B, np.uint8).view(vals.dtype)
+    ek, ev = expected(keys, vals, sb, se, init_k, init_v, descending, begin_bit, end_bit)
+    u = f"u{keys.itemsize}"
+    assert np.array_equal(bk[sel].view(u), ek.view(u)), "keys differ"
+    if vals is not None:
+        assert np.array_equal(bv[sel].view(np.uint8), ev.view(np.uint8)), "values differ"
+
+
+@pytest.mark.parametrize("descending", [False, True])
+@pytest.mark.parametrize("overwrite", [False, True])
+def test_lognormal_mixed_classes(descending, overwrite):
+    rng = np.random.default_rng(4 + descending + 2 * overwrite)
+    sizes = lognormal_segments(rng, 3000, 512, 2.0, 1, 1 << 17)
+    sb, se = contiguous_offsets(sizes)
+    n = int(se[-1])
+    keys = random_keys(rng, np.uint32, n)
+    vals = np.arange(n, dtype=np.uint32)
+    check(keys, vals, sb, se, descending, overwrite=overwrite)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.float32, np.int64, np.float64])
+def test_key_types_keys_only(dtype):
+    rng = np.random.default_rng(8)
+    sizes = lognormal_segments(rng, 500, 300, 1.5, 0, 40000)
+    sb, se = contiguous_offsets(sizes)
+    n = int(se[-1])
+    keys = random_keys(rng, dtype, n)
+    for desc in (False, True):
+        check(keys, None, sb, se, desc)
+
+
+def test_unspecified_ranges_and_empty_segments():
+    """Gaps between segments stay untouched; empty and reversed ranges are no-ops."""
+    rng = np.random.default_rng(21)
+    n = 200000
+    keys = random_keys(rng, np.uint32, n, "and2")
+    vals = np.arange(n, dtype=np.uint32)
+    sb = np.array([0, 10, 5000, 5000, 60000, 100000, 199990, 120000])
+    se = np.array([5, 4000, 5000, 50000, 90000, 100000, 200000, 119000])  # [2] empty, [7] reversed
+    for ow in (False, True):
+        check(keys, vals, sb, se, overwrite=ow)
+
+
+@pytest.mark.parametrize("offset_dtype", [np.int32, np.int64])
+def test_offset_widths(offset_dtype):
+    rng = np.random.default_rng(3)
+    sizes = rng.integers(0, 20000, 64)
+    sb, se = contiguous_offsets(sizes)
+    n = int(se[-1])
+    keys = random_keys(rng, np.int32, n)
+    vals = np.arange(n, dtype=np.uint64)
+    check(keys, vals, sb, se, offset_dtype=offset_dtype)
+
+
+@pytest.mark.parametrize("window", [(0, 7), (3, 19), (16, 32), (5, 5)])
+def test_bit_windows(window):
+    rng = np.random.default_rng(31)
+    sizes = lognormal_segments(rng, 200, 2000, 1.5, 1, 60000)
+    sb, se = contiguous_offsets(sizes)
+    n = int(se[-1])
+    keys = random_keys(rng, np.uint32, n)
+    vals = np.arange(n, dtype=np.uint32)
+    b, e = window
+    for desc in (False, True):
+        check(keys, vals, sb, se, desc, b, e)
+
+
+def test_begin_equals_end_with_overwrite_is_noop():
+    import cccl_amd as ca
+
+    n = 1000
+    keys = np.arange(n, dtype=np.uint32)[::-1].copy()
+    kd = to_dev(keys)
+    ko = torch.zeros_like(kd.view(torch.int32)).view(torch.uint32)
+    sb = to_dev(np.array([0], np.int32))
+    se = to_dev(np.array([n], np.int32))
+    s = ca.SegmentedSorter(torch.uint32, 0, n, 1, begin_bit=4, end_bit=4, overwrite=True)
+    assert s.sort(kd, ko, sb, se) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(kd, np.uint32), keys)
+
+
+def _cfg(classes, large, strategy=0):
+    from cccl_amd import _bindings as B
+
+    c = B.segmented_config()
+    c.num_block_classes = len(classes)
+    for i, (t, ipt) in enumerate(classes):
+        c.block_threads[i] = t
+        c.block_ipt[i] = ipt
+    c.large_threads, c.large_ipt = large
+    c.strategy = strategy
+    return c
+
+
+@pytest.mark.parametrize("large", [(256, 16), (512, 8)])
+def test_per_segment_pass_schedule(large):
+    """The reference-equivalent schedule (one workgroup per segment per pass)."""
+    rng = np.random.default_rng(55)
+    sizes = lognormal_segments(rng, 800, 512, 2.0, 0, 100000)
+    sb, se = contiguous_offsets(sizes)
+    n = int(se[-1])
+    keys = random_keys(rng, np.float32, n)
+    vals = np.arange(n, dtype=np.uint32)
+    for desc, ow in ((False, False), (True, True)):
+        check(keys, vals, sb, se, descending=desc, overwrite=ow, config=_cfg([], large, strategy=1))
+    check(keys, None, sb, se, begin_bit=3, end_bit=21, config=_cfg([], large, strategy=1))
+
+
+@pytest.mark.parametrize("classes,large", [
+    ([], (512, 16)),                                   # everything through the batched onesweep
+    ([(256, 4)], (256, 16)),
+    ([(256, 8), (512, 16)], (512, 8)),
+    ([(256, 4), (256, 8), (256, 16), (512, 16)], (512, 16)),
+])
+def test_class_routings(classes, large):
+    rng = np.random.default_rng(len(classes))
+    sizes = lognormal_segments(rng, 1500, 512, 2.0, 1, 300000)
+    sb, se = contiguous_offsets(sizes)
+    n = int(se[-1])
+    keys = random_keys(rng, np.uint32, n, "and3")
+    vals = np.arange(n, dtype=np.uint32)
+    check(keys, vals, sb, se, config=_cfg(classes, large))
+    check(keys, vals, sb, se, descending=True, overwrite=True, config=_cfg(classes, large))
+
+
+def test_two_large_segments():
+    """catch2_test_device_segmented_radix_sort_keys.cu: two very large segments."""
+    rng = np.random.default_rng(77)
+    n = 3_000_000
+    keys = random_keys(rng, np.uint32, n)
+    vals = np.arange(n, dtype=np.uint32)
+    sb = np.array([0, 1_700_001])
+    se = np.array([1_700_001, n])
+    check(keys, vals, sb, se)
+
+
+def test_many_tiny_segments():
+    rng = np.random.default_rng(78)
+    sizes = rng.integers(0, 5, 200000)
+    sb, se = contiguous_offsets(sizes)
+    n = int(se[-1])
+    keys = random_keys(rng, np.uint16, n)
+    vals = np.arange(n, dtype=np.uint32)
+    check(keys, vals, sb, se, descending=True)
