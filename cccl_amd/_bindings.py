@@ -149,6 +149,17 @@ EXPORTED_SYMBOLS = [
     "cccl_amd_block_sort_shapes",
     "cccl_amd_profile_enable",
     "cccl_amd_profile_read",
+    "cccl_amd_capture_begin",
+    "cccl_amd_capture_end",
+    "cccl_amd_graph_launch",
+    "cccl_amd_graph_destroy",
+    "cccl_amd_last_error_message",
+    "cccl_amd_device_info",
+    "cccl_amd_sort_features",
+    "cccl_amd_mlp_param_count",
+    "cccl_amd_mlp_workspace_bytes",
+    "cccl_amd_mlp_forward",
+    "cccl_amd_mlp_train_step",
 ]
 
 _lib = None
@@ -206,6 +217,30 @@ def lib():
     L.cccl_amd_profile_enable.restype = None
     L.cccl_amd_profile_read.argtypes = [ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_float), i32]
     L.cccl_amd_profile_read.restype = i32
+    L.cccl_amd_capture_begin.argtypes = [vp]
+    L.cccl_amd_capture_begin.restype = i32
+    L.cccl_amd_capture_end.argtypes = [vp, ctypes.POINTER(vp)]
+    L.cccl_amd_capture_end.restype = i32
+    L.cccl_amd_graph_launch.argtypes = [vp, vp]
+    L.cccl_amd_graph_launch.restype = i32
+    L.cccl_amd_graph_destroy.argtypes = [vp]
+    L.cccl_amd_graph_destroy.restype = i32
+    L.cccl_amd_last_error_message.argtypes = []
+    L.cccl_amd_last_error_message.restype = ctypes.c_char_p
+    L.cccl_amd_device_info.argtypes = [ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                       ctypes.c_char_p, i32]
+    L.cccl_amd_device_info.restype = i32
+    fp = ctypes.c_void_p
+    L.cccl_amd_sort_features.argtypes = [vp, u64, i32, sz, i32, i32, i32, i64, vp, vp, i32, fp, vp]
+    L.cccl_amd_sort_features.restype = i32
+    L.cccl_amd_mlp_param_count.argtypes = [i32]
+    L.cccl_amd_mlp_param_count.restype = i32
+    L.cccl_amd_mlp_workspace_bytes.argtypes = [i32, i32]
+    L.cccl_amd_mlp_workspace_bytes.restype = sz
+    L.cccl_amd_mlp_forward.argtypes = [fp, fp, i32, i32, fp, vp]
+    L.cccl_amd_mlp_forward.restype = i32
+    L.cccl_amd_mlp_train_step.argtypes = [fp, fp, fp, fp, fp, fp, sz, fp, fp, fp, i32, i32, ctypes.c_float, i32, vp]
+    L.cccl_amd_mlp_train_step.restype = i32
     _lib = L
     return L
 

@@ -27,6 +27,9 @@ SOURCES = [
     "dispatch.hip",
     "segmented.hip",
     "capi.hip",
+    "graph.hip",
+    "mlp.hip",
+    "features.hip",
 ]
 
 # -packed-fp32-ops: no v_pk_*_f32 generation anywhere in the device code.

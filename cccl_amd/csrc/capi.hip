@@ -22,49 +22,9 @@ namespace {
 using cccl_amd::DoubleBufferRef;
 using cccl_amd::SortProblem;
 
-// Key type -> (bytes, kind).  FLOAT16 covers any 16-bit IEEE-style float (fp16, bf16):
-// both transform alike (sign bit on top).
 bool key_traits(cccl_type_enum t, int& bytes, int& kind)
 {
-  switch (t)
-  {
-    case CCCL_INT8:
-      bytes = 1, kind = cccl_amd::kSignedKey;
-      return true;
-    case CCCL_INT16:
-      bytes = 2, kind = cccl_amd::kSignedKey;
-      return true;
-    case CCCL_INT32:
-      bytes = 4, kind = cccl_amd::kSignedKey;
-      return true;
-    case CCCL_INT64:
-      bytes = 8, kind = cccl_amd::kSignedKey;
-      return true;
-    case CCCL_UINT8:
-    case CCCL_BOOLEAN:
-      bytes = 1, kind = cccl_amd::kUnsignedKey;
-      return true;
-    case CCCL_UINT16:
-      bytes = 2, kind = cccl_amd::kUnsignedKey;
-      return true;
-    case CCCL_UINT32:
-      bytes = 4, kind = cccl_amd::kUnsignedKey;
-      return true;
-    case CCCL_UINT64:
-      bytes = 8, kind = cccl_amd::kUnsignedKey;
-      return true;
-    case CCCL_FLOAT16:
-      bytes = 2, kind = cccl_amd::kFloatKey;
-      return true;
-    case CCCL_FLOAT32:
-      bytes = 4, kind = cccl_amd::kFloatKey;
-      return true;
-    case CCCL_FLOAT64:
-      bytes = 8, kind = cccl_amd::kFloatKey;
-      return true;
-    default:
-      return false;
-  }
+  return cccl_amd::key_traits_from_enum(int(t), bytes, kind);
 }
 
 bool value_size_ok(size_t v)
@@ -851,6 +811,42 @@ CCCL_C_API int cccl_amd_onesweep_shapes(int* threads, int* ipt, int max)
     ipt[k]     = i[k];
   }
   return n;
+}
+
+CCCL_C_API const char* cccl_amd_last_error_message(void)
+{
+  return cccl_amd::last_error();
+}
+
+CCCL_C_API int cccl_amd_device_info(int* num_cus, int* max_lds_per_block, int* lds_per_cu, char* arch, int arch_len)
+{
+  int dev = 0;
+  if (hipError_t e = hipGetDevice(&dev); e != hipSuccess)
+  {
+    return int(e);
+  }
+  hipDeviceProp_t prop{};
+  if (hipError_t e = hipGetDeviceProperties(&prop, dev); e != hipSuccess)
+  {
+    return int(e);
+  }
+  if (num_cus != nullptr)
+  {
+    *num_cus = prop.multiProcessorCount;
+  }
+  if (max_lds_per_block != nullptr)
+  {
+    *max_lds_per_block = int(prop.sharedMemPerBlock);
+  }
+  if (lds_per_cu != nullptr)
+  {
+    *lds_per_cu = int(prop.maxSharedMemoryPerMultiProcessor);
+  }
+  if (arch != nullptr && arch_len > 0)
+  {
+    std::snprintf(arch, size_t(arch_len), "%s", prop.gcnArchName);
+  }
+  return hipSuccess;
 }
 
 CCCL_C_API int cccl_amd_block_sort_shapes(int* threads, int* ipt, int max)

@@ -1,6 +1,8 @@
 // Host dispatch of the device-wide radix sort: temp-storage plan, launch sequence,
 // DoubleBuffer bookkeeping, kernel timing hooks.
 #include <algorithm>
+#include <cstdarg>
+#include <cstdio>
 #include <mutex>
 #include <set>
 #include <utility>
@@ -96,6 +98,7 @@ hipError_t ensure_lds_attribute(const void* fn, size_t lds)
   }
   if (hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)); e != hipSuccess)
   {
+    set_last_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize, %zu) failed: %s", lds, hipGetErrorString(e));
     return e;
   }
   done.insert({dev, fn});
@@ -175,11 +178,33 @@ hipError_t launch_kernel(const void* fn, uint64_t grid, int threads, size_t lds,
   profile_record_begin(stream, kind);
   hipError_t e = hipLaunchKernel(fn, dim3(uint32_t(grid)), dim3(uint32_t(threads)), args, lds, stream);
   profile_record_end(stream);
+  if (e == hipSuccess)
+  {
+    e = hipGetLastError();
+  }
   if (e != hipSuccess)
   {
-    return e;
+    set_last_error("launch of kernel kind %d failed: %s (grid %llu, threads %d, lds %zu bytes)", kind,
+                   hipGetErrorString(e), static_cast<unsigned long long>(grid), threads, lds);
   }
-  return hipGetLastError();
+  return e;
+}
+
+namespace {
+thread_local char g_last_error[512] = {0};
+} // namespace
+
+void set_last_error(const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+}
+
+const char* last_error()
+{
+  return g_last_error;
 }
 
 hipError_t alias_temporaries(void* d_temp, size_t& temp_bytes, void** parts, const size_t* sizes, int count)
@@ -200,6 +225,7 @@ hipError_t alias_temporaries(void* d_temp, size_t& temp_bytes, void** parts, con
   }
   if (temp_bytes < needed)
   {
+    set_last_error("temp storage of %zu bytes is smaller than the %zu bytes needed", temp_bytes, needed);
     return hipErrorInvalidValue;
   }
   const uintptr_t base = (reinterpret_cast<uintptr_t>(d_temp) + kAlign - 1) & ~uintptr_t(kAlign - 1);
@@ -208,6 +234,49 @@ hipError_t alias_temporaries(void* d_temp, size_t& temp_bytes, void** parts, con
     parts[i] = reinterpret_cast<void*>(base + offsets[i]);
   }
   return hipSuccess;
+}
+
+bool key_traits_from_enum(int type, int& bytes, int& kind)
+{
+  switch (type)
+  {
+    case CCCL_INT8:
+      bytes = 1, kind = kSignedKey;
+      return true;
+    case CCCL_INT16:
+      bytes = 2, kind = kSignedKey;
+      return true;
+    case CCCL_INT32:
+      bytes = 4, kind = kSignedKey;
+      return true;
+    case CCCL_INT64:
+      bytes = 8, kind = kSignedKey;
+      return true;
+    case CCCL_UINT8:
+    case CCCL_BOOLEAN:
+      bytes = 1, kind = kUnsignedKey;
+      return true;
+    case CCCL_UINT16:
+      bytes = 2, kind = kUnsignedKey;
+      return true;
+    case CCCL_UINT32:
+      bytes = 4, kind = kUnsignedKey;
+      return true;
+    case CCCL_UINT64:
+      bytes = 8, kind = kUnsignedKey;
+      return true;
+    case CCCL_FLOAT16:
+      bytes = 2, kind = kFloatKey;
+      return true;
+    case CCCL_FLOAT32:
+      bytes = 4, kind = kFloatKey;
+      return true;
+    case CCCL_FLOAT64:
+      bytes = 8, kind = kFloatKey;
+      return true;
+    default:
+      return false;
+  }
 }
 
 // Twiddle constants of a key type.

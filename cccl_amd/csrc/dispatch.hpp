@@ -89,6 +89,14 @@ hipError_t launch_kernel(const void* fn, uint64_t grid, int threads, size_t lds,
 
 void twiddle_constants(int key_bytes, int key_kind, bool descending, uint64_t& xor_mask, int& float_mode);
 
+// cccl_type_enum value -> (key bytes, KeyKind); false for types that cannot be keys.
+// FLOAT16 covers any 16-bit sign-magnitude float (fp16, bf16): both transform alike.
+bool key_traits_from_enum(int type, int& bytes, int& kind);
+
+// Thread-local description of the last failure inside the library.
+void set_last_error(const char* fmt, ...);
+const char* last_error();
+
 // Optional per-launch timing used by benchmarks.
 void profile_record_begin(hipStream_t stream, int kind);
 void profile_record_end(hipStream_t stream);

@@ -1,0 +1,239 @@
+// Input statistics of one sort call for the config selector, computed on the device by
+// one workgroup: segment-size mix from the offset arrays and digit entropies from a
+// strided sample of the keys.  Cost: one small launch and a 64-byte copy, far below
+// the sort itself for the sizes the selector is used on.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "../../include/cccl_amd/config_selector.h"
+#include "dispatch.hpp"
+#include "radix_common.hpp"
+
+namespace cccl_amd {
+namespace {
+
+constexpr int kFeatThreads = 1024;
+constexpr int kSample      = 4096;
+constexpr int kMaxPassF    = 8;
+
+struct FeatureParams
+{
+  const void* keys;
+  uint64_t num_items;
+  const void* begin_offsets;
+  const void* end_offsets;
+  int64_t num_segments;
+  uint64_t xor_mask;
+  float* out;
+  int key_bytes;
+  int float_mode;
+  int begin_bit;
+  int end_bit;
+  int offset_bytes;
+  int value_bytes;
+  int descending;
+  int pad_;
+};
+
+template <class UK>
+__device__ void sample_entropies(const FeatureParams& p, uint32_t* s_hist, float* s_red, float* ent, int passes)
+{
+  const UK* keys = static_cast<const UK*>(p.keys);
+  const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
+  const uint64_t n = p.num_items;
+  const int S      = n < uint64_t(kSample) ? int(n) : kSample;
+  for (int i = threadIdx.x; i < kMaxPassF * kRadix; i += kFeatThreads)
+  {
+    s_hist[i] = 0;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < S; i += kFeatThreads)
+  {
+    const uint64_t idx = (uint64_t(i) * n) / uint64_t(S);
+    const UK t         = tw(keys[idx]);
+    for (int q = 0; q < passes; ++q)
+    {
+      const int shift     = p.begin_bit + q * kRadixBits;
+      const int bits      = min(kRadixBits, p.end_bit - shift);
+      const uint32_t mask = (1u << bits) - 1u;
+      atomicAdd(&s_hist[q * kRadix + extract_digit(t, shift, mask)], 1u);
+    }
+  }
+  __syncthreads();
+  // entropy of each pass's digit distribution, in bits / 8 (1.0 = uniform 8-bit digits)
+  for (int q = 0; q < passes; ++q)
+  {
+    float h = 0.f;
+    for (int d = threadIdx.x; d < kRadix; d += kFeatThreads)
+    {
+      const float c = float(s_hist[q * kRadix + d]);
+      if (c > 0.f)
+      {
+        const float pr = c / float(S);
+        h -= pr * log2f(pr);
+      }
+    }
+    s_red[threadIdx.x] = h;
+    __syncthreads();
+    for (int w = kFeatThreads / 2; w > 0; w >>= 1)
+    {
+      if (int(threadIdx.x) < w)
+      {
+        s_red[threadIdx.x] += s_red[threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+      ent[q] = s_red[0] / 8.f;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kFeatThreads) features_kernel(FeatureParams p)
+{
+  __shared__ uint32_t s_hist[kMaxPassF * kRadix];
+  __shared__ float s_red[kFeatThreads];
+  __shared__ double s_seg[4][kFeatThreads / 64];
+  __shared__ float s_ent[kMaxPassF];
+
+  // segment-size mix: items in segments of <= 1Ki, (1Ki, 8Ki], (8Ki, 64Ki], > 64Ki
+  double acc[4] = {0, 0, 0, 0};
+  for (int64_t s = threadIdx.x; s < p.num_segments; s += kFeatThreads)
+  {
+    int64_t b, e;
+    if (p.offset_bytes == 4)
+    {
+      b = static_cast<const int32_t*>(p.begin_offsets)[s];
+      e = static_cast<const int32_t*>(p.end_offsets)[s];
+    }
+    else
+    {
+      b = static_cast<const int64_t*>(p.begin_offsets)[s];
+      e = static_cast<const int64_t*>(p.end_offsets)[s];
+    }
+    const double sz = e > b ? double(e - b) : 0.0;
+    acc[sz <= 1024.0 ? 0 : sz <= 8192.0 ? 1 : sz <= 65536.0 ? 2 : 3] += sz;
+  }
+  for (int k = 0; k < 4; ++k)
+  {
+    double v = acc[k];
+    for (int o = 32; o > 0; o >>= 1)
+    {
+      v += __shfl_down(v, o, 64);
+    }
+    if ((threadIdx.x & 63u) == 0)
+    {
+      s_seg[k][threadIdx.x >> 6] = v;
+    }
+  }
+  __syncthreads();
+
+  const int passes = max(1, (p.end_bit - p.begin_bit + kRadixBits - 1) / kRadixBits);
+  if (p.num_items > 0 && p.end_bit > p.begin_bit)
+  {
+    switch (p.key_bytes)
+    {
+      case 1:
+        sample_entropies<uint8_t>(p, s_hist, s_red, s_ent, passes);
+        break;
+      case 2:
+        sample_entropies<uint16_t>(p, s_hist, s_red, s_ent, passes);
+        break;
+      case 4:
+        sample_entropies<uint32_t>(p, s_hist, s_red, s_ent, passes);
+        break;
+      default:
+        sample_entropies<uint64_t>(p, s_hist, s_red, s_ent, passes);
+        break;
+    }
+  }
+  else if (threadIdx.x < kMaxPassF)
+  {
+    s_ent[threadIdx.x] = 0.f;
+  }
+  __syncthreads();
+
+  if (threadIdx.x == 0)
+  {
+    double seg[4] = {0, 0, 0, 0}, tot = 0;
+    for (int k = 0; k < 4; ++k)
+    {
+      for (int w = 0; w < kFeatThreads / 64; ++w)
+      {
+        seg[k] += s_seg[k][w];
+      }
+      tot += seg[k];
+    }
+    const bool segmented = p.begin_offsets != nullptr;
+    const float n        = float(p.num_items);
+    float mean = 0.f, mn = 1.f;
+    for (int q = 0; q < passes; ++q)
+    {
+      mean += s_ent[q];
+      mn = fminf(mn, s_ent[q]);
+    }
+    mean /= float(passes);
+    float* o = p.out;
+    o[0]     = log2f(fmaxf(n, 1.f)) / 28.f;
+    o[1]     = float(p.key_bytes) / 8.f;
+    o[2]     = float(p.value_bytes) / 16.f;
+    o[3]     = p.float_mode ? 1.f : 0.f;
+    o[4]     = p.descending ? 1.f : 0.f;
+    o[5]     = float(p.end_bit - p.begin_bit) / 64.f;
+    o[6]     = segmented ? 1.f : 0.f;
+    o[7]     = segmented ? log2f(float(p.num_segments) + 1.f) / 20.f : 0.f;
+    for (int k = 0; k < 4; ++k)
+    {
+      o[8 + k] = segmented && tot > 0 ? float(seg[k] / tot) : (k == 3 ? 1.f : 0.f);
+    }
+    o[12] = mean;
+    o[13] = mn;
+    o[14] = s_ent[passes - 1];
+    o[15] = segmented ? log2f(fmaxf(n / fmaxf(float(p.num_segments), 1.f), 1.f)) / 28.f : o[0];
+  }
+}
+
+} // namespace
+} // namespace cccl_amd
+
+extern "C" CCCL_C_API int cccl_amd_sort_features(
+  const void* d_keys,
+  uint64_t num_items,
+  cccl_type_enum key_type,
+  size_t value_bytes,
+  int begin_bit,
+  int end_bit,
+  int descending,
+  int64_t num_segments,
+  const void* d_begin_offsets,
+  const void* d_end_offsets,
+  int offset_bytes,
+  float* features_dev,
+  CUstream stream)
+{
+  int kb = 0, kind = 0;
+  if (!cccl_amd::key_traits_from_enum(int(key_type), kb, kind) || features_dev == nullptr || begin_bit < 0
+      || end_bit > kb * 8 || begin_bit > end_bit || (d_begin_offsets != nullptr && offset_bytes != 4 && offset_bytes != 8))
+  {
+    return hipErrorInvalidValue;
+  }
+  cccl_amd::FeatureParams p{};
+  p.keys          = d_keys;
+  p.num_items     = num_items;
+  p.begin_offsets = d_begin_offsets;
+  p.end_offsets   = d_end_offsets;
+  p.num_segments  = d_begin_offsets != nullptr ? num_segments : 0;
+  p.out           = features_dev;
+  p.key_bytes     = kb;
+  p.begin_bit     = begin_bit;
+  p.end_bit       = end_bit;
+  p.offset_bytes  = offset_bytes;
+  p.value_bytes   = int(value_bytes);
+  p.descending    = descending;
+  cccl_amd::twiddle_constants(kb, kind, descending != 0, p.xor_mask, p.float_mode);
+  hipLaunchKernelGGL(cccl_amd::features_kernel, dim3(1), dim3(cccl_amd::kFeatThreads), 0,
+                     reinterpret_cast<hipStream_t>(stream), p);
+  return int(hipGetLastError());
+}

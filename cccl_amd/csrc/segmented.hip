@@ -190,11 +190,14 @@ hipError_t segmented_radix_sort(
     cls[c] = find_entry(tab->block_sort, tab->num_block_sort, kb, vb, cfg.block_threads[c], cfg.block_ipt[c]);
     if (cls[c] == nullptr)
     {
+      set_last_error("segmented: no one-workgroup kernel %dx%d for %d-byte keys, %d-byte values", cfg.block_threads[c],
+                     cfg.block_ipt[c], kb, vb);
       return hipErrorInvalidValue;
     }
     class_max[c] = uint32_t(cls[c]->threads * cls[c]->ipt);
     if (c > 0 && class_max[c] <= class_max[c - 1])
     {
+      set_last_error("segmented: class tiles must increase");
       return hipErrorInvalidValue;
     }
   }
@@ -202,6 +205,8 @@ hipError_t segmented_radix_sort(
     find_entry(tab->seg_onesweep, tab->num_seg_onesweep, kb, vb, cfg.large_threads, cfg.large_ipt);
   if (large == nullptr)
   {
+    set_last_error("segmented: no batched onesweep kernel %dx%d for %d-byte keys, %d-byte values", cfg.large_threads,
+                   cfg.large_ipt, kb, vb);
     return hipErrorInvalidValue;
   }
 

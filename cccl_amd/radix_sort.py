@@ -46,8 +46,21 @@ class DoubleBuffer:
 
 class SortError(RuntimeError):
     def __init__(self, where: str, code: int):
-        super().__init__(f"{where} failed with code {code}")
+        try:
+            detail = (B.lib().cccl_amd_last_error_message() or b"").decode()
+        except Exception:
+            detail = ""
+        super().__init__(f"{where} failed with code {code}" + (f": {detail}" if detail else ""))
         self.code = code
+
+
+def device_info() -> dict:
+    n, lds_blk, lds_cu = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    arch = ctypes.create_string_buffer(64)
+    rc = B.lib().cccl_amd_device_info(ctypes.byref(n), ctypes.byref(lds_blk), ctypes.byref(lds_cu), arch, 64)
+    if rc != 0:
+        raise SortError("cccl_amd_device_info", rc)
+    return {"cus": n.value, "max_lds_per_block": lds_blk.value, "lds_per_cu": lds_cu.value, "arch": arch.value.decode()}
 
 
 _KEY_TYPES = {

@@ -1,0 +1,224 @@
+"""Launch-configuration arms, input features, and the learned config selector.
+
+The sort arithmetic never depends on the choice: every arm produces the same bits
+(tests/test_gpu_sort.py::test_every_onesweep_shape and friends); arms differ in speed
+only.  The selector is a 16 -> 64 -> 64 -> C MLP (cccl_amd/csrc/mlp.hip) that predicts,
+per arm, log(time / best time) of a call from 16 cheap features
+(cccl_amd/csrc/features.hip) and picks the argmin over the arms valid for the call.
+It is trained on timings produced by the exhaustive autotuner (cccl_amd/autotune.py),
+which also serves as its fallback and as the judge of its regret.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+from dataclasses import dataclass
+from pathlib import Path
+
+import torch
+
+from . import _bindings as B
+from .radix_sort import SegmentedSorter, SortError, Sorter, _stream_handle, make_config
+
+NUM_FEATURES = 16
+WEIGHTS_PATH = Path(__file__).resolve().parent / "selector_weights.json"
+
+
+@dataclass(frozen=True)
+class Arm:
+    """One launch configuration.  kind 'plain' or 'segmented'."""
+    name: str
+    kind: str
+    onesweep: tuple[int, int] = (512, 16)
+    single_tile: tuple[int, int] = (256, 16)  # (0, 0) disables the one-workgroup path
+    block_classes: tuple[tuple[int, int], ...] = ()
+    large: tuple[int, int] = (512, 16)
+    strategy: int = 0
+
+    def plain_config(self) -> B.radix_sort_config:
+        return make_config(self.onesweep[0], self.onesweep[1], single_tile_threads=self.single_tile[0],
+                           single_tile_ipt=self.single_tile[1])
+
+    def segmented_config(self) -> B.segmented_config:
+        c = B.segmented_config()
+        c.num_block_classes = len(self.block_classes)
+        for i, (t, ipt) in enumerate(self.block_classes):
+            c.block_threads[i], c.block_ipt[i] = t, ipt
+        c.large_threads, c.large_ipt = self.large
+        c.strategy = self.strategy
+        return c
+
+
+def _plain_arms() -> list[Arm]:
+    arms = []
+    for os_shape in [(256, 8), (256, 16), (512, 8), (512, 12), (512, 16)]:
+        for st in [(0, 0), (256, 16), (512, 16)]:
+            arms.append(Arm(f"plain os{os_shape[0]}x{os_shape[1]} st{st[0] * st[1]}", "plain", onesweep=os_shape,
+                            single_tile=st))
+    return arms
+
+
+def _segmented_arms() -> list[Arm]:
+    arms = []
+    class_sets = {
+        "c4": ((256, 4), (256, 8), (256, 16), (512, 16)),
+        "c2": ((256, 8), (512, 16)),
+        "c1": ((256, 16),),
+        "c0": (),
+    }
+    for cname, classes in class_sets.items():
+        for large in [(256, 16), (512, 8), (512, 16)]:
+            arms.append(Arm(f"seg {cname} L{large[0]}x{large[1]}", "segmented", block_classes=classes, large=large))
+    for large in [(256, 16), (512, 8)]:
+        arms.append(Arm(f"seg per-segment-pass {large[0]}x{large[1]}", "segmented", large=large, strategy=1))
+    return arms
+
+
+ARMS: list[Arm] = _plain_arms() + _segmented_arms()
+NUM_ARMS = len(ARMS)
+
+# Fixed defaults a caller gets without tuning: CUB-like fixed shapes (one onesweep shape
+# for every size; CUB's one-workgroup-per-segment-per-pass schedule for segmented calls).
+FIXED_DEFAULT = {"plain": ARMS.index(next(a for a in ARMS if a.name == "plain os512x16 st4096")),
+                 "segmented": ARMS.index(next(a for a in ARMS if a.name == "seg per-segment-pass 256x16"))}
+
+
+def arm_valid(arm: Arm, key_bytes: int, value_bytes: int, segmented: bool) -> bool:
+    """Whether the arm applies to the call and its kernels exist for these widths."""
+    from .radix_sort import config_valid
+
+    if (arm.kind == "segmented") != segmented:
+        return False
+    item = key_bytes + value_bytes
+    if arm.kind == "plain":
+        return config_valid(arm.plain_config(), key_bytes, value_bytes)
+    # LDS limit of the widest tiles (check by item width: the tile of keys + values)
+    for t, ipt in arm.block_classes + (arm.large,):
+        if t * ipt * item + 16 * 1024 > 160 * 1024:
+            return False
+    return True
+
+
+def valid_mask(key_bytes: int, value_bytes: int, segmented: bool) -> list[bool]:
+    return [arm_valid(a, key_bytes, value_bytes, segmented) for a in ARMS]
+
+
+# ---- features ---------------------------------------------------------------------
+
+
+def features(keys, num_items: int, key_type: int, value_bytes: int, begin_bit: int, end_bit: int, descending: bool,
+             seg_begin=None, seg_end=None, num_segments: int = 0, out=None, stream=None):
+    """16 features on the device (torch float32 tensor)."""
+    if out is None:
+        out = torch.empty(NUM_FEATURES, dtype=torch.float32, device=keys.device)
+    rc = B.lib().cccl_amd_sort_features(
+        keys.data_ptr(), num_items, key_type, value_bytes, begin_bit, end_bit, int(descending), num_segments,
+        None if seg_begin is None else seg_begin.data_ptr(), None if seg_end is None else seg_end.data_ptr(),
+        0 if seg_begin is None else seg_begin.element_size(), out.data_ptr(), _stream_handle(stream))
+    if rc != 0:
+        raise SortError("cccl_amd_sort_features", rc)
+    return out
+
+
+# ---- the network --------------------------------------------------------------------
+
+
+class MLPSelector:
+    """Config-selector network on the device; training and inference run the library's
+    HIP kernels (cccl_amd_mlp_train_step / cccl_amd_mlp_forward)."""
+
+    def __init__(self, num_out: int = NUM_ARMS, seed: int = 0, device="cuda"):
+        self.num_out = num_out
+        self.device = device
+        self.P = B.lib().cccl_amd_mlp_param_count(num_out)
+        assert self.P > 0
+        self.params = torch.from_numpy(self.init_params(num_out, seed)).to(device)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.grad = torch.zeros_like(self.params)
+        self.loss = torch.zeros(1, dtype=torch.float32, device=device)
+        self.step = 0
+
+    @staticmethod
+    def init_params(num_out: int, seed: int):
+        import numpy as np
+
+        rng = np.random.default_rng(seed)
+        parts = []
+        for fan_in, fan_out in ((16, 64), (64, 64), (64, num_out)):
+            parts.append((rng.standard_normal((fan_in, fan_out)) * math.sqrt(2.0 / fan_in)).astype(np.float32).ravel())
+            parts.append(np.zeros(fan_out, np.float32))
+        return np.concatenate(parts)
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        x = x.contiguous().float()
+        y = torch.empty((x.shape[0], self.num_out), dtype=torch.float32, device=x.device)
+        rc = B.lib().cccl_amd_mlp_forward(self.params.data_ptr(), x.data_ptr(), x.shape[0], self.num_out,
+                                          y.data_ptr(), _stream_handle(None))
+        if rc != 0:
+            raise SortError("cccl_amd_mlp_forward", rc)
+        return y
+
+    def gradient(self, x, target, mask):
+        """Gradient of the masked mean squared error (no update)."""
+        self._train_step(x, target, mask, lr=0.0, step=0)
+        return self.grad.clone(), float(self.loss.item())
+
+    def _train_step(self, x, target, mask, lr, step):
+        x, target, mask = x.contiguous().float(), target.contiguous().float(), mask.contiguous().float()
+        nbytes = B.lib().cccl_amd_mlp_workspace_bytes(x.shape[0], self.num_out)
+        ws = torch.empty(max(nbytes // 4, 1), dtype=torch.float32, device=x.device)
+        rc = B.lib().cccl_amd_mlp_train_step(self.params.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
+                                             self.grad.data_ptr(), self.loss.data_ptr(), ws.data_ptr(), nbytes,
+                                             x.data_ptr(), target.data_ptr(), mask.data_ptr(), x.shape[0],
+                                             self.num_out, lr, step, _stream_handle(None))
+        if rc != 0:
+            raise SortError("cccl_amd_mlp_train_step", rc)
+
+    def fit(self, x, target, mask, epochs=600, lr=3e-3, batch=256, seed=0) -> list[float]:
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        n = x.shape[0]
+        losses = []
+        for ep in range(epochs):
+            perm = torch.randperm(n, generator=g).to(x.device)
+            for s in range(0, n, batch):
+                idx = perm[s:s + batch]
+                self.step += 1
+                self._train_step(x[idx], target[idx], mask[idx], lr, self.step)
+            if ep % 50 == 0 or ep == epochs - 1:
+                self._train_step(x, target, mask, 0.0, 0)
+                losses.append(float(self.loss.item()))
+        return losses
+
+    def choose(self, x: torch.Tensor, valid: torch.Tensor) -> torch.Tensor:
+        """Arm index per row: argmin predicted cost over the valid arms."""
+        y = self.forward(x)
+        y = torch.where(valid, y, torch.full_like(y, float("inf")))
+        return torch.argmin(y, dim=1)
+
+    def save(self, path: Path = WEIGHTS_PATH, meta: dict | None = None):
+        blob = {"version": 1, "arms": [a.name for a in ARMS], "num_out": self.num_out,
+                "params": self.params.cpu().tolist(), "meta": meta or {}}
+        Path(path).write_text(json.dumps(blob))
+
+    @classmethod
+    def load(cls, path: Path = WEIGHTS_PATH, device="cuda") -> "MLPSelector":
+        blob = json.loads(Path(path).read_text())
+        if blob.get("arms") != [a.name for a in ARMS]:
+            raise ValueError("selector weights were trained for a different arm list")
+        sel = cls(blob["num_out"], device=device)
+        sel.params = torch.tensor(blob["params"], dtype=torch.float32, device=device)
+        return sel
+
+
+# ---- running an arm -----------------------------------------------------------------
+
+
+def make_sorter(arm: Arm, key_dtype, value_bytes, n, descending, begin_bit, end_bit, num_segments=0, offset_bytes=4):
+    if arm.kind == "plain":
+        return Sorter(key_dtype, value_bytes, n, descending=descending, begin_bit=begin_bit, end_bit=end_bit,
+                      config=arm.plain_config())
+    return SegmentedSorter(key_dtype, value_bytes, n, num_segments, offset_bytes=offset_bytes, descending=descending,
+                           begin_bit=begin_bit, end_bit=end_bit, config=arm.segmented_config())

@@ -1,0 +1,73 @@
+/* Learned launch-configuration selector of the MI355X radix sort: input statistics and
+ * the config-selector network's HIP kernels.  No counterpart in the reference, whose
+ * configuration comes from compile-time policy tables
+ * (cub/cub/device/dispatch/tuning/tuning_radix_sort.cuh:1724-2360).
+ *
+ * All entry points are stream-ordered and return hipError_t codes. */
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifndef CCCL_C_EXPERIMENTAL
+#  define CCCL_C_EXPERIMENTAL
+#endif
+#include <cccl/c/types.h>
+
+CCCL_C_EXTERN_C_BEGIN
+
+#define CCCL_AMD_NUM_FEATURES 16
+
+/* Writes the 16 selector features of one sort call to features_dev (16 floats of device
+ * memory): call shape (log2 n, key/value width, float keys, order, bit count, segment
+ * count and size mix) and digit entropies of a strided sample of up to 4,096 keys.
+ * d_begin_offsets == NULL describes an unsegmented call. */
+CCCL_C_API int cccl_amd_sort_features(
+  const void* d_keys,
+  uint64_t num_items,
+  cccl_type_enum key_type,
+  size_t value_bytes,
+  int begin_bit,
+  int end_bit,
+  int descending,
+  int64_t num_segments,
+  const void* d_begin_offsets,
+  const void* d_end_offsets,
+  int offset_bytes,
+  float* features_dev,
+  CUstream stream);
+
+/* Config-selector network 16 -> 64 -> 64 -> out (ReLU), fp32 parameters in one array of
+ * cccl_amd_mlp_param_count(out) floats: W1[16][64], b1[64], W2[64][64], b2[64],
+ * W3[64][out], b3[out].  out <= 64. */
+CCCL_C_API int cccl_amd_mlp_param_count(int out);
+
+/* Bytes of the per-call workspace cccl_amd_mlp_train_step needs for `batch` samples. */
+CCCL_C_API size_t cccl_amd_mlp_workspace_bytes(int batch, int out);
+
+/* y[batch][out] = network(x[batch][16]). */
+CCCL_C_API int cccl_amd_mlp_forward(const float* params, const float* x, int batch, int out, float* y, CUstream stream);
+
+/* One training step on the masked squared error sum(mask * (y - target)^2) / sum(mask):
+ * writes the gradient of that mean loss to grad (param_count floats) and the loss to
+ * loss[0] (device, may be NULL); with step >= 1 also applies Adam step `step` (lr,
+ * beta1 0.9, beta2 0.999, eps 1e-8) to params using adam_m / adam_v; step 0 only computes
+ * the gradient.  The gradient reduction is deterministic. */
+CCCL_C_API int cccl_amd_mlp_train_step(
+  float* params,
+  float* adam_m,
+  float* adam_v,
+  float* grad,
+  float* loss,
+  float* workspace,
+  size_t workspace_bytes,
+  const float* x,
+  const float* target,
+  const float* mask,
+  int batch,
+  int out,
+  float lr,
+  int step,
+  CUstream stream);
+
+CCCL_C_EXTERN_C_END

@@ -120,6 +120,24 @@ CCCL_C_API int cccl_amd_radix_sort_config_valid(const cccl_amd_radix_sort_config
 CCCL_C_API int cccl_amd_onesweep_shapes(int* threads, int* ipt, int max);
 CCCL_C_API int cccl_amd_block_sort_shapes(int* threads, int* ipt, int max);
 
+/* Text describing the last failure of a cccl_amd_* call on this thread ("" if none). */
+CCCL_C_API const char* cccl_amd_last_error_message(void);
+
+/* Current device: compute units, LDS bytes one workgroup may use, LDS bytes per CU,
+ * gfx architecture name (any pointer may be NULL). */
+CCCL_C_API int cccl_amd_device_info(int* num_cus, int* max_lds_per_block, int* lds_per_cu, char* arch, int arch_len);
+
+/* hipGraph capture of sort calls.  Between begin and end, calls issued on `stream`
+ * (which must not be the legacy default stream) are recorded, not run; end instantiates
+ * the recording into *graph_exec, which graph_launch replays on any stream.  The sort
+ * entry points issue only kernel launches on their stream (no memset, no host sync), so
+ * a replay repeats the recorded calls with the recorded pointers and sizes; temp storage
+ * must stay allocated while the graph lives.  Calls that overlap on one temp buffer race. */
+CCCL_C_API int cccl_amd_capture_begin(CUstream stream);
+CCCL_C_API int cccl_amd_capture_end(CUstream stream, void** graph_exec);
+CCCL_C_API int cccl_amd_graph_launch(void* graph_exec, CUstream stream);
+CCCL_C_API int cccl_amd_graph_destroy(void* graph_exec);
+
 /* Kernel timing for benchmarks: while enabled, every kernel the sorts launch is
  * bracketed by a pair of HIP events on its own stream.  read waits for the recorded
  * events, writes up to max (kind, milliseconds) pairs and returns how many there were.
