@@ -14,6 +14,7 @@ from .radix_sort import (  # noqa: F401
     config_valid,
     default_config,
     default_segmented_config,
+    device_info,
     make_config,
     make_radix_sort,
     onesweep_shapes,

@@ -1,0 +1,280 @@
+"""Exhaustive autotuner and learned-selector experiment for the MI355X radix sort.
+
+For each sort call it times every launch-configuration arm valid for the call (HIP
+events on the sort's stream, median of a few runs after a warm-up) and records the
+call's 16 selector features.  The records train the config-selector network
+(cccl_amd.selector.MLPSelector, trained by the library's HIP kernels) and judge it:
+regret = time of the arm it picks / time of the exhaustive best - 1.
+
+Stream of calls (the learned-selection workload): n log-uniform in [2^10, 2^28], key
+type uniform over {u8, u16, u32, u64, f32}, entropy one of six levels (uniform, AND of
+2..5 uniform words, all-equal), 30% segmented with lognormal sizes (median 512,
+sigma 2.0, clipped to [1, 2^21]); uint32 values = index.
+
+    python -m cccl_amd.autotune stream --calls 2000 --out timings.json
+    python -m cccl_amd.autotune train --timings timings.json --weights cccl_amd/selector_weights.json
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import statistics
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import selector as S
+from .radix_sort import key_type_of
+
+KEY_DTYPES = {"u8": torch.uint8, "u16": torch.uint16, "u32": torch.uint32, "u64": torch.uint64, "f32": torch.float32}
+ENTROPIES = ["uniform", "and2", "and3", "and4", "and5", "equal"]
+
+
+def make_stream(count: int, seed: int, log2_min=10, log2_max=28, seg_frac=0.3):
+    rng = np.random.default_rng(seed)
+    calls = []
+    for i in range(count):
+        n = int(round(2 ** rng.uniform(log2_min, log2_max)))
+        calls.append({
+            "id": i,
+            "n": n,
+            "key": str(rng.choice(list(KEY_DTYPES))),
+            "entropy": str(rng.choice(ENTROPIES)),
+            "segmented": bool(rng.random() < seg_frac),
+            "seed": int(rng.integers(0, 2**31)),
+        })
+    return calls
+
+
+def lognormal_offsets(n: int, seed: int, median=512.0, sigma=2.0, lo=1, hi=1 << 21):
+    """Contiguous segments with lognormal sizes covering exactly n items."""
+    rng = np.random.default_rng(seed)
+    sizes = []
+    total = 0
+    while total < n:
+        k = max(16, int((n - total) / median) + 16)
+        s = np.clip(np.rint(np.exp(np.log(median) + sigma * rng.standard_normal(k))), lo, hi).astype(np.int64)
+        c = np.cumsum(s)
+        cut = int(np.searchsorted(c, n - total))
+        if cut < k:
+            s = s[: cut + 1].copy()
+            s[-1] -= int(c[cut] - (n - total))
+            s = s[s > 0]
+            sizes.append(s)
+            total = n
+        else:
+            sizes.append(s)
+            total += int(c[-1])
+    sizes = np.concatenate(sizes)
+    ends = np.cumsum(sizes)
+    return (ends - sizes).astype(np.int32), ends.astype(np.int32)
+
+
+class Buffers:
+    """Device buffers reused across calls (largest call decides the size)."""
+
+    def __init__(self, max_n: int, device):
+        self.device = device
+        self.keys_in = torch.empty(max_n * 8, dtype=torch.uint8, device=device)
+        self.keys_out = torch.empty(max_n * 8, dtype=torch.uint8, device=device)
+        self.vals_in = torch.arange(max_n, dtype=torch.int32, device=device)
+        self.vals_out = torch.empty(max_n, dtype=torch.int32, device=device)
+
+    def keys(self, dtype, n, which="in"):
+        b = self.keys_in if which == "in" else self.keys_out
+        es = torch.empty(0, dtype=dtype).element_size()
+        return b[: n * es].view(dtype)
+
+
+def fill_keys(buf: Buffers, call) -> torch.Tensor:
+    dtype = KEY_DTYPES[call["key"]]
+    n = call["n"]
+    es = torch.empty(0, dtype=dtype).element_size()
+    g = torch.Generator(device=buf.device).manual_seed(call["seed"])
+    raw = buf.keys_in[: n * es]
+    words = raw.view(torch.uint8)
+
+    def rand_bytes():
+        return torch.randint(0, 256, (n * es,), dtype=torch.uint8, device=buf.device, generator=g)
+
+    ent = call["entropy"]
+    if call["key"] == "f32" and ent == "uniform":
+        raw.view(torch.float32).copy_(torch.randn(n, device=buf.device, generator=g))
+    elif ent == "equal":
+        words.view(-1, es).copy_(rand_bytes()[:es].view(1, es).expand(n, es))
+    else:
+        k = {"uniform": 1, "and2": 2, "and3": 3, "and4": 4, "and5": 5}[ent]
+        words.copy_(rand_bytes())
+        for _ in range(k - 1):
+            words.bitwise_and_(rand_bytes())
+    return raw.view(dtype)
+
+
+def time_fn(fn, reps: int) -> float:
+    fn()  # warm-up (also allocates nothing: sorters own their temp storage)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    return statistics.median(ts)
+
+
+def reps_for(n: int) -> int:
+    return 7 if n < (1 << 16) else 5 if n < (1 << 20) else 3 if n < (1 << 24) else 2
+
+
+def measure_call(call, buf: Buffers, features_out=None, verify=False):
+    """Features and per-arm times (ms; None for arms not valid for the call)."""
+    dtype = KEY_DTYPES[call["key"]]
+    n = call["n"]
+    kin = fill_keys(buf, call)
+    kout = buf.keys(dtype, n, "out")
+    vin, vout = buf.vals_in[:n], buf.vals_out[:n]
+    kb = kin.element_size()
+    seg = call["segmented"]
+    sb = se = None
+    nseg = 0
+    if seg:
+        b, e = lognormal_offsets(n, call["seed"] ^ 0x5EED)
+        sb = torch.from_numpy(b).to(buf.device)
+        se = torch.from_numpy(e).to(buf.device)
+        nseg = int(b.size)
+    feat = S.features(kin, n, key_type_of(dtype), 4, 0, kb * 8, False, sb, se, nseg)
+    t_feat = time_fn(lambda: S.features(kin, n, key_type_of(dtype), 4, 0, kb * 8, False, sb, se, nseg, out=feat), 3)
+    valid = S.valid_mask(kb, 4, seg)
+    times = [None] * S.NUM_ARMS
+    ref = None
+    for i, arm in enumerate(S.ARMS):
+        if not valid[i]:
+            continue
+        srt = S.make_sorter(arm, dtype, 4, n, False, 0, kb * 8, nseg)
+        if seg:
+            fn = lambda: srt.sort(kin, kout, sb, se, vin, vout)  # noqa: E731
+        else:
+            fn = lambda: srt.sort(kin, kout, vin, vout)  # noqa: E731
+        times[i] = time_fn(fn, reps_for(n))
+        if verify:
+            out = vout.clone()
+            if ref is None:
+                ref = out
+            elif not torch.equal(ref, out):
+                raise AssertionError(f"arm {arm.name} gave a different permutation for call {call['id']}")
+        del srt
+    return {"call": call, "features": feat.cpu().tolist(), "feature_ms": t_feat, "times_ms": times,
+            "valid": valid, "num_segments": nseg}
+
+
+def run_stream(calls, out_path, device="cuda", verify_every=50, log=sys.stdout):
+    max_n = max(c["n"] for c in calls)
+    buf = Buffers(max_n, torch.device(device))
+    records = []
+    t0 = time.time()
+    for i, call in enumerate(calls):
+        rec = measure_call(call, buf, verify=(i % verify_every == 0))
+        records.append(rec)
+        if i % 20 == 0 or i == len(calls) - 1:
+            print(f"[autotune] {i + 1}/{len(calls)} calls, {time.time() - t0:.0f}s", file=log, flush=True)
+            with open(out_path, "w") as f:
+                json.dump({"arms": [a.name for a in S.ARMS], "records": records}, f)
+    return records
+
+
+# ---- selector training and judging -------------------------------------------------
+
+
+def dataset(records):
+    X = torch.tensor([r["features"] for r in records], dtype=torch.float32)
+    T = torch.zeros((len(records), S.NUM_ARMS), dtype=torch.float32)
+    M = torch.zeros_like(T)
+    best = []
+    for i, r in enumerate(records):
+        ts = [t for t in r["times_ms"] if t is not None]
+        b = min(ts)
+        best.append(b)
+        for j, t in enumerate(r["times_ms"]):
+            if t is not None:
+                T[i, j] = math.log(max(t, 1e-4) / max(b, 1e-4))
+                M[i, j] = 1.0
+    return X, T, M
+
+
+def judge(records, choice) -> dict:
+    """Totals over the calls: chosen arms, fixed default, exhaustive best."""
+    tot_sel = tot_def = tot_best = tot_feat = 0.0
+    worst = 0.0
+    for r, c in zip(records, choice):
+        ts = r["times_ms"]
+        kind = "segmented" if r["call"]["segmented"] else "plain"
+        best = min(t for t in ts if t is not None)
+        t_sel = ts[c]
+        tot_sel += t_sel
+        tot_feat += r["feature_ms"]
+        tot_def += ts[S.FIXED_DEFAULT[kind]]
+        tot_best += best
+        worst = max(worst, t_sel / best - 1)
+    return {"calls": len(records), "selected_ms": tot_sel, "feature_ms": tot_feat, "default_ms": tot_def,
+            "exhaustive_ms": tot_best, "speedup_vs_default": tot_def / (tot_sel + tot_feat),
+            "speedup_vs_default_excl_features": tot_def / tot_sel,
+            "regret_vs_exhaustive": (tot_sel + tot_feat) / tot_best - 1,
+            "regret_vs_exhaustive_excl_features": tot_sel / tot_best - 1, "worst_call_regret": worst}
+
+
+def train_and_judge(records, train_frac=0.7, epochs=600, seed=0, device="cuda"):
+    n_train = int(len(records) * train_frac)
+    X, T, M = dataset(records)
+    X, T, M = X.to(device), T.to(device), M.to(device)
+    tr, te = slice(0, n_train), slice(n_train, None)
+    net = S.MLPSelector(seed=seed, device=device)
+    untrained = S.MLPSelector(seed=seed, device=device)
+    losses = net.fit(X[tr], T[tr], M[tr], epochs=epochs, seed=seed)
+    valid = M > 0
+    pick = net.choose(X[te], valid[te]).cpu().tolist()
+    pick_untrained = untrained.choose(X[te], valid[te]).cpu().tolist()
+    held = records[n_train:]
+    out = {"train_calls": n_train, "test_calls": len(held), "losses": losses,
+           "learned": judge(held, pick), "untrained": judge(held, pick_untrained)}
+    # a hand rule a caller might write: our built-in default shapes
+    return net, out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("stream")
+    a.add_argument("--calls", type=int, default=2000)
+    a.add_argument("--seed", type=int, default=5)
+    a.add_argument("--log2-max", type=int, default=28)
+    a.add_argument("--out", required=True)
+    b = sub.add_parser("train")
+    b.add_argument("--timings", required=True)
+    b.add_argument("--weights", default=str(S.WEIGHTS_PATH))
+    b.add_argument("--epochs", type=int, default=600)
+    b.add_argument("--report", default=None)
+    args = ap.parse_args(argv)
+    if args.cmd == "stream":
+        calls = make_stream(args.calls, args.seed, log2_max=args.log2_max)
+        run_stream(calls, args.out)
+        return 0
+    data = json.loads(open(args.timings).read())
+    if data["arms"] != [x.name for x in S.ARMS]:
+        raise SystemExit("timings were recorded for a different arm list")
+    net, rep = train_and_judge(data["records"], epochs=args.epochs)
+    net.save(args.weights, meta={"trained_on": args.timings, "report": rep["learned"]})
+    txt = json.dumps(rep, indent=1)
+    if args.report:
+        open(args.report, "w").write(txt)
+    print(txt)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

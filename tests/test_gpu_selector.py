@@ -1,0 +1,116 @@
+"""The config selector's HIP kernels against the float64 CPU network, the feature
+kernel against host-computed statistics, and what training buys."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import mlp_ref
+from sortutil import to_dev
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _net(out, seed=1):
+    from cccl_amd.selector import MLPSelector
+
+    return MLPSelector(num_out=out, seed=seed)
+
+
+@pytest.mark.parametrize("batch,out", [(1, 1), (37, 5), (300, 31), (1000, 64)])
+def test_forward_matches_cpu(batch, out):
+    rng = np.random.default_rng(batch)
+    net = _net(out)
+    x = rng.standard_normal((batch, 16)).astype(np.float32)
+    y = net.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    yr, _ = mlp_ref.forward(net.params.cpu().numpy().astype(np.float64), x.astype(np.float64), out)
+    assert np.allclose(y, yr, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("batch,out", [(33, 3), (257, 29), (1024, 64)])
+def test_gradient_matches_cpu_float64(batch, out):
+    """fp32 device gradient vs float64 backprop (pinned by finite differences): relative
+    error of the whole gradient vector below 1e-4."""
+    rng = np.random.default_rng(out)
+    net = _net(out, seed=3)
+    x = rng.standard_normal((batch, 16)).astype(np.float32)
+    t = rng.standard_normal((batch, out)).astype(np.float32)
+    m = (rng.random((batch, out)) > 0.25).astype(np.float32)
+    g, loss = net.gradient(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda(), torch.from_numpy(m).cuda())
+    p64 = net.params.cpu().numpy().astype(np.float64)
+    gr = mlp_ref.grad(p64, x, t, m, out)
+    lr = mlp_ref.loss(p64, x, t, m, out)
+    g = g.cpu().numpy()
+    assert np.linalg.norm(g - gr) <= 1e-4 * np.linalg.norm(gr)
+    assert abs(loss - lr) <= 1e-4 * abs(lr)
+
+
+def test_gradient_is_deterministic():
+    rng = np.random.default_rng(2)
+    net = _net(20)
+    x = torch.from_numpy(rng.standard_normal((500, 16)).astype(np.float32)).cuda()
+    t = torch.from_numpy(rng.standard_normal((500, 20)).astype(np.float32)).cuda()
+    m = torch.ones_like(t)
+    g1, _ = net.gradient(x, t, m)
+    g2, _ = net.gradient(x, t, m)
+    assert torch.equal(g1, g2)
+
+
+def test_training_beats_untrained_and_fixed_choice():
+    """Synthetic cost table where the best of 8 arms depends on the features; the trained
+    network's pick has far lower regret than a random-init network's or a fixed arm's."""
+    from cccl_amd.selector import MLPSelector
+
+    rng = np.random.default_rng(7)
+    n, C = 3000, 8
+    x = rng.random((n, 16)).astype(np.float32)
+    centers = np.linspace(0, 1, C)
+    cost = np.abs(x[:, :1] - centers[None, :]) * 4 + np.abs(x[:, 1:2] - 0.5)  # best arm ~ x0
+    t = (cost - cost.min(1, keepdims=True)).astype(np.float32)
+    m = np.ones_like(t)
+    X, T, M = (torch.from_numpy(a).cuda() for a in (x, t, m))
+    net = MLPSelector(num_out=C, seed=0)
+    untrained = MLPSelector(num_out=C, seed=0)
+    losses = net.fit(X[:2000], T[:2000], M[:2000], epochs=80, lr=3e-3)
+    assert losses[-1] < 0.2 * losses[0]
+    valid = M[2000:] > 0
+
+    def regret(pick):
+        c = cost[2000:]
+        return float((c[np.arange(1000), pick] / c.min(1)).mean() - 1)
+
+    r_trained = regret(net.choose(X[2000:], valid).cpu().numpy())
+    r_untrained = regret(untrained.choose(X[2000:], valid).cpu().numpy())
+    r_fixed = min(regret(np.full(1000, a)) for a in range(C))
+    assert r_trained < 0.05
+    assert r_trained < 0.25 * r_untrained and r_trained < 0.25 * r_fixed
+
+
+def test_features_match_host_statistics():
+    from cccl_amd import selector as S
+    from cccl_amd.radix_sort import key_type_of
+
+    rng = np.random.default_rng(4)
+    n = 1 << 20
+    keys = (rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+            & rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32))
+    kd = to_dev(keys)
+    sizes = np.array([100, 5000, 70000, n - 75100])
+    ends = np.cumsum(sizes).astype(np.int32)
+    begins = (ends - sizes).astype(np.int32)
+    f = S.features(kd, n, key_type_of(torch.uint32), 4, 0, 32, False, to_dev(begins), to_dev(ends), 4).cpu().numpy()
+    assert abs(f[0] - 20 / 28) < 1e-6 and f[1] == 0.5 and f[2] == 0.25 and f[6] == 1.0
+    assert np.allclose(f[8:12], sizes / n, atol=1e-6)
+    # sampled digit entropy of AND-2 keys: bits with P(1) = 1/4 -> 8 * H(0.25) bits / 8
+    h = -(0.25 * math.log2(0.25) + 0.75 * math.log2(0.75))
+    assert abs(f[12] - h) < 0.05
+    f2 = S.features(kd, n, key_type_of(torch.uint32), 0, 0, 32, True).cpu().numpy()
+    assert f2[6] == 0.0 and f2[4] == 1.0 and f2[11] == 1.0
