@@ -686,9 +686,11 @@ CCCL_C_API int cccl_amd_radix_sort(
   {
     return hipErrorNotSupported;
   }
-  if (vals0 == nullptr)
+  // value_bytes alone decides keys-only, so a size query may pass null buffers
+  if (value_bytes > 0 && d_temp_storage != nullptr && (vals0 == nullptr || vals1 == nullptr))
   {
-    value_bytes = 0;
+    cccl_amd::set_last_error("value_bytes is %zu but a value buffer is null", value_bytes);
+    return hipErrorInvalidValue;
   }
   pr.val_bytes  = int(value_bytes);
   pr.num_items  = num_items;
@@ -741,9 +743,11 @@ CCCL_C_API int cccl_amd_segmented_radix_sort(
   {
     return hipErrorNotSupported;
   }
-  if (vals0 == nullptr)
+  // value_bytes alone decides keys-only, so a size query may pass null buffers
+  if (value_bytes > 0 && d_temp_storage != nullptr && (vals0 == nullptr || vals1 == nullptr))
   {
-    value_bytes = 0;
+    cccl_amd::set_last_error("value_bytes is %zu but a value buffer is null", value_bytes);
+    return hipErrorInvalidValue;
   }
   pr.val_bytes  = int(value_bytes);
   pr.num_items  = num_items;

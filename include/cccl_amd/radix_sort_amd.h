@@ -57,8 +57,9 @@ typedef struct cccl_amd_segmented_config_t
  * [Descending] (cub/cub/device/device_radix_sort.cuh:411-449, :1126-1154, :1775, :3034,
  * :4211) and its dispatch (cub/cub/device/dispatch/dispatch_radix_sort.cuh:2008-2065).
  * keys0/keys1 with *selector form a cub::DoubleBuffer; the pointer API is
- * is_overwrite_okay = 0 with keys0 = input and keys1 = output.  vals0 == NULL sorts keys
- * only.  value_bytes must be 1, 2, 4, 8 or 16 (or 0 for keys only).  cfg == NULL picks
+ * is_overwrite_okay = 0 with keys0 = input and keys1 = output.  value_bytes 0 sorts keys
+ * only; otherwise it must be 1, 2, 4, 8 or 16 and vals0/vals1 non-null (a size query,
+ * d_temp_storage == NULL, may pass null buffers).  cfg == NULL picks
  * the built-in default.  On return *selector names the buffer holding the result: with
  * is_overwrite_okay = 0 that is 1 (keys1) whenever the call sorted or copied anything. */
 CCCL_C_API int cccl_amd_radix_sort(
