@@ -117,6 +117,8 @@ def main() -> int:
     ap.add_argument("--entropies", default=",".join(ENTROPIES))
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=25)
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="counter runs: one 2 GiB device copy (reference bytes), then --steps sorts only")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -136,6 +138,21 @@ def main() -> int:
     vout = torch.empty(n, dtype=torch.int32, device=dev).view(torch.uint32)
     sorter = ca.Sorter(torch.uint32, 4, n)
     cfg = sorter.config if sorter.config is not None else ca.default_config(4, 4, n)
+
+    if args.pmc_workload:
+        # A plain copy of known size first: its FETCH_SIZE / WRITE_SIZE give the scale
+        # factor between counter units and bytes; then the sorts being measured.
+        src = torch.empty(1 << 28, dtype=torch.int64, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        kin = gen_keys(torch, n, "uniform", 42, dev)
+        for _ in range(args.steps):
+            sorter.sort(kin, kout, vin, vout)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_workload": "copy 2 GiB int64, then sorts", "sorts": args.steps, "n": n}), flush=True)
+        return 0
 
     rates = {}
     parity = {}

@@ -489,6 +489,13 @@ hipError_t radix_sort(
     {
       return e;
     }
+    void* args = d_bins;
+    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&digit_starts_kernel), uint64_t(passes), kRadix, 0,
+                                     stream, &args, 8);
+        e != hipSuccess)
+    {
+      return e;
+    }
   }
 
   void* d_keys_tmp = keys.alternate();
@@ -525,7 +532,7 @@ hipError_t radix_sort(
       op.float_mode      = float_mode;
       op.shift           = bit;
       op.num_bits        = num_bits;
-      op.bins_are_starts = portion == 0 ? 0 : 1;
+      op.small_offsets   = n * uint64_t(std::max(kb, vb)) < (uint64_t(1) << 32) ? 1 : 0;
       if (hipError_t e = launch_kernel(os->fn, tiles, os->threads, os->lds_bytes, stream, &op, 2); e != hipSuccess)
       {
         return e;

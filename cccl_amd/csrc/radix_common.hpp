@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace cccl_amd {
 
 constexpr int kRadixBits = 8;
@@ -62,6 +64,29 @@ struct Twiddle
   }
 };
 
+// Same transform with the key kind fixed at compile time: integer keys cost one xor,
+// floating-point keys a compare, a select, an arithmetic shift and two xors.
+template <class UK, bool FLOAT>
+struct TwiddleT
+{
+  UK xor_mask;
+
+  static constexpr int kBits = int(sizeof(UK) * 8);
+  static constexpr UK kHigh  = UK(UK(1) << (kBits - 1));
+
+  __device__ __forceinline__ UK operator()(UK k) const
+  {
+    if constexpr (FLOAT)
+    {
+      k = (k == kHigh) ? UK(0) : k; // -0.0 reads as +0.0
+      using S     = typename std::make_signed<UK>::type;
+      const UK neg = UK(S(k) >> (kBits - 1)); // all ones for negatives
+      k           = UK(k ^ UK(neg | kHigh));
+    }
+    return UK(k ^ xor_mask);
+  }
+};
+
 template <class UK>
 __host__ __device__ __forceinline__ uint32_t extract_digit(UK t, int shift, uint32_t mask)
 {
@@ -83,25 +108,27 @@ __device__ __forceinline__ uint32_t mask_prefix_count(uint64_t m)
   return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
 
-// Set of lanes of the wave whose 8-bit digit equals this lane's: one ballot per bit;
-// a lane keeps the lanes that agree with it on that bit.  With s = this lane's bit
-// sign-extended to 0 / ~0, the lanes agreeing on the bit are ~(ballot ^ s), so each
-// bit costs one bit-field extract, one compare and an xnor + and per 32-lane half.
+// Set of lanes of the wave whose 8-bit digit equals this lane's: one ballot per bit.
+// With s = this lane's bit sign-extended to 0 / ~0, the lanes that differ from it on
+// that bit are ballot ^ s; OR-ing those over the bits and complementing leaves the
+// peers.  gfx950's three-input v_bitop3 does "acc | (ballot ^ s)" in one instruction
+// per 32-lane half (truth table 0xf6 over (acc, ballot, s)), so a bit costs one
+// v_bfe_i32, one v_cmp and two v_bitop3.  (The bfe / icmp intrinsics keep the compiler
+// from re-deriving the compare from a shifted copy of d, which costs a shift per bit.)
 template <int BITS>
 __device__ __forceinline__ uint64_t wave_match_digit(uint32_t d)
 {
-  const uint64_t exec = __builtin_amdgcn_read_exec();
-  uint32_t lo         = uint32_t(exec);
-  uint32_t hi         = uint32_t(exec >> 32);
+  uint32_t lo = 0, hi = 0;
 #pragma unroll
   for (int b = 0; b < BITS; ++b)
   {
-    const uint32_t s = uint32_t(int32_t(d << (31 - b)) >> 31); // bit b of d as 0 / ~0
-    const uint64_t m = __builtin_amdgcn_ballot_w64(s != 0u);
-    lo &= ~(uint32_t(m) ^ s);
-    hi &= ~(uint32_t(m >> 32) ^ s);
+    const uint32_t s = uint32_t(__builtin_amdgcn_sbfe(int(d), b, 1)); // bit b of d as 0 / ~0
+    const uint64_t m = __builtin_amdgcn_uicmp(s, 0u, 33 /* ICMP_NE */);
+    lo               = __builtin_amdgcn_bitop3_b32(lo, uint32_t(m), s, 0xf6);
+    hi               = __builtin_amdgcn_bitop3_b32(hi, uint32_t(m >> 32), s, 0xf6);
   }
-  return (uint64_t(hi) << 32) | lo;
+  const uint64_t exec = __builtin_amdgcn_read_exec();
+  return exec & ~((uint64_t(hi) << 32) | lo);
 }
 
 // Inclusive scan across the 64 lanes of a wave.
@@ -119,6 +146,22 @@ __device__ __forceinline__ T wave_inclusive_scan(T v)
     }
   }
   return v;
+}
+
+// 32-bit inclusive scan with DPP lane moves: four row shifts scan each row of 16 lanes,
+// then row_bcast:15 and row_bcast:31 carry row totals into the rows above.  Lanes a
+// move cannot source (and rows the row mask leaves out) add 0.
+template <>
+__device__ __forceinline__ uint32_t wave_inclusive_scan<uint32_t>(uint32_t v)
+{
+  int x = int(v);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false); // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false); // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false); // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false); // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false); // row_bcast:15 into rows 1, 3
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false); // row_bcast:31 into rows 2, 3
+  return uint32_t(x);
 }
 
 // Exclusive scan of one value per thread over the first kRadix threads of a block
@@ -184,7 +227,8 @@ __device__ __forceinline__ void lds_store_wave(uint32_t* p, uint32_t v)
 // Stable rank of IPT digits per lane within one wave.  Item i of lane l stands at
 // position i*64 + l of the wave's strip, so items are visited in position order:
 // slot by slot, and within a slot by lane.  wc[d] counts the digit-d items seen so
-// far; the highest peer lane writes the new count.
+// far.  Every peer lane writes the same new count to wc[d] (same address, same value
+// within one store instruction), which needs no leader election or branch.
 template <int IPT>
 __device__ __forceinline__ void wave_rank(const uint32_t (&dig)[IPT], uint32_t (&rank)[IPT], uint32_t* wc)
 {
@@ -196,10 +240,37 @@ __device__ __forceinline__ void wave_rank(const uint32_t (&dig)[IPT], uint32_t (
     const uint32_t cnt   = uint32_t(__builtin_popcountll(peers));
     const uint32_t old   = lds_load_wave(wc + dig[i]);
     rank[i]              = old + below;
-    if (below + 1u == cnt)
-    {
-      lds_store_wave(wc + dig[i], old + cnt);
-    }
+    lds_store_wave(wc + dig[i], old + cnt);
+  }
+}
+
+// wave_rank with the digits produced on the fly by digit_of(i) and each result packed
+// as rank | digit << 16 (in-wave ranks stay below 2^16), so no digit array stays live.
+template <int IPT, class DigitOf>
+__device__ __forceinline__ void wave_rank_packed(uint32_t (&rd)[IPT], uint32_t* wc, DigitOf digit_of)
+{
+#pragma unroll
+  for (int i = 0; i < IPT; ++i)
+  {
+    const uint32_t d     = digit_of(i);
+    const uint64_t peers = wave_match_digit<kRadixBits>(d);
+    const uint32_t below = mask_prefix_count(peers);
+    const uint32_t cnt   = uint32_t(__builtin_popcountll(peers));
+    const uint32_t old   = lds_load_wave(wc + d);
+    rd[i]                = (old + below) | (d << 16);
+    lds_store_wave(wc + d, old + cnt);
+  }
+}
+
+// Zeroes one wave's kRadix counters; only that wave touches them until the next
+// workgroup barrier, so no barrier is needed before its rank.
+__device__ __forceinline__ void zero_wave_counters(uint32_t* wc)
+{
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int j = 0; j < kRadix / 64; ++j)
+  {
+    lds_store_wave(wc + j * 64 + lane, 0u);
   }
 }
 

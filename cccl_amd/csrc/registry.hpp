@@ -49,9 +49,11 @@ const KeyWidthTable* kernels_for_key_bytes(int key_bytes);
 #define CCCL_AMD_ONESWEEP_SHAPES(X) \
   X(256, 8)                         \
   X(256, 16)                        \
+  X(384, 16)                        \
   X(512, 8)                         \
   X(512, 12)                        \
-  X(512, 16)
+  X(512, 16)                        \
+  X(1024, 8)
 
 // Launch shapes compiled for the batched onesweep of large segments.
 #define CCCL_AMD_SEG_ONESWEEP_SHAPES(X) \

@@ -399,6 +399,7 @@ hipError_t segmented_radix_sort(
       op.total_tiles     = d_counters + kSegLargeTiles;
       op.pass            = pass;
       op.num_passes      = passes;
+      op.small_offsets   = n * uint64_t(std::max(kb, vb)) < (uint64_t(1) << 32) ? 1 : 0;
       if (hipError_t e = launch_kernel(large->fn, tile_bound, large->threads, large->lds_bytes, stream, &op, 6);
           e != hipSuccess)
       {

@@ -5,11 +5,11 @@
 //   histogram_kernel  one read of the keys builds the digit histogram of every pass
 //                     (role of DeviceRadixSortHistogramKernel, kernel_radix_sort.cuh:447-473,
 //                     agent_radix_sort_histogram.cuh:205-246)
+//   digit_starts_kernel exclusive sum of each pass's digit counts (role of
+//                     DeviceRadixSortExclusiveSumKernel, kernel_radix_sort.cuh:563-606)
 //   onesweep_kernel   one digit pass: wave-ballot rank, decoupled look-back over tiles,
 //                     LDS-staged scatter (role of DeviceRadixSortOnesweepKernel,
-//                     kernel_radix_sort.cuh:498-558, agent_radix_sort_onesweep.cuh:667-697);
-//                     the per-pass exclusive sum of the bins (ExclusiveSumKernel,
-//                     kernel_radix_sort.cuh:563-606) is folded into every tile
+//                     kernel_radix_sort.cuh:498-558, agent_radix_sort_onesweep.cuh:667-697)
 //   block_sort_kernel whole LSD sort of up to one tile inside one workgroup, one
 //                     workgroup per segment (role of DeviceRadixSortSingleTileKernel,
 //                     kernel_radix_sort.cuh:330-434, and of the per-segment kernel
@@ -47,6 +47,17 @@ static __global__ void __launch_bounds__(256) init_kernel(InitParams p)
   {
     p.lookback[i] = 0;
   }
+}
+
+// Turns the digit counts of every pass (bins[pass][kRadix], one workgroup per pass) into
+// exclusive starts in place, so each onesweep tile reads its digit starts directly.
+static __global__ void __launch_bounds__(kRadix) digit_starts_kernel(uint64_t* bins)
+{
+  __shared__ uint64_t s_wave[kRadix / 64];
+  uint64_t* b        = bins + uint64_t(blockIdx.x) * kRadix;
+  uint64_t all       = 0;
+  const uint64_t ex  = radix_exclusive_scan<uint64_t>(b[threadIdx.x], s_wave, all);
+  b[threadIdx.x]     = ex;
 }
 
 struct HistogramParams
@@ -159,7 +170,7 @@ struct OnesweepParams
   uint32_t* lookback; // [tiles][kRadix], zero on entry
   uint32_t* lookback_next; // cleared here for the next launch (may be null)
   uint32_t* tile_counter; // zero on entry
-  const uint64_t* bins; // plain: digit counts of the whole input (bins_are_starts == 0) or digit starts of this portion
+  const uint64_t* bins; // plain: first output position of every digit for this portion
   uint64_t* bins_next; // plain: last tile writes digit starts of the next portion (may be null)
   uint64_t num_items; // plain: items in this portion (< 2^30)
   uint64_t xor_mask;
@@ -168,7 +179,8 @@ struct OnesweepParams
   int float_mode;
   int shift;
   int num_bits;
-  int bins_are_starts;
+  int small_offsets; // output arrays below 4 GiB: 32-bit byte offsets
+
   // SEG only
   const uint32_t* tile_slot; // segment slot of each tile
   const uint32_t* slot_first_tile; // first tile of each slot
@@ -182,23 +194,82 @@ struct OnesweepParams
 template <class UK, class V, int THREADS, int IPT>
 struct OnesweepLayout
 {
-  static constexpr int kWaves      = THREADS / 64;
-  static constexpr int kTile       = THREADS * IPT;
-  static constexpr bool kHasV      = !std::is_same<V, NoValue>::value;
-  static constexpr size_t kWcnt    = size_t(kWaves) * kRadix * 4;
-  static constexpr size_t kBase    = size_t(kRadix) * 8;
-  static constexpr size_t kMisc    = 64;
-  static constexpr size_t kKeys    = size_t(kTile) * sizeof(UK);
-  static constexpr size_t kVals    = kHasV ? size_t(kTile) * sizeof(V) : 0;
-  static constexpr size_t kOffBase = kWcnt;
-  static constexpr size_t kOffMisc = kOffBase + kBase;
-  static constexpr size_t kOffKeys = kOffMisc + kMisc;
-  static constexpr size_t kOffVals = (kOffKeys + kKeys + 15) & ~size_t(15);
-  static constexpr size_t kBytes   = kOffVals + kVals;
+  static constexpr int kWaves       = THREADS / 64;
+  static constexpr int kTile        = THREADS * IPT;
+  static constexpr bool kHasV       = !std::is_same<V, NoValue>::value;
+  static constexpr size_t kWcnt     = size_t(kWaves) * kRadix * 4;
+  static constexpr size_t kBase     = size_t(kRadix) * 8;
+  static constexpr size_t kMisc     = 64;
+  static constexpr size_t kOffBase  = kWcnt;
+  static constexpr size_t kOffMisc  = kOffBase + kBase;
+  static constexpr size_t kOffKeys  = kOffMisc + kMisc;
+  static constexpr size_t kOffVals  = kOffKeys + size_t(kTile) * sizeof(UK);
+  static constexpr size_t kBytes    = kOffVals + (kHasV ? size_t(kTile) * sizeof(V) : 0);
 };
 
-template <class UK, class V, int THREADS, int IPT, bool SEG>
-__global__ void __launch_bounds__(THREADS) onesweep_kernel(OnesweepParams p)
+// Sum of digit d's counts over tiles [first_tile, tile) from the look-back status words.
+// Each round trip reads kLookbackWindow predecessors at once (independent loads), so a
+// walk over many tiles that have only published their own counts costs a few round
+// trips to the coherence point instead of one per tile.  A word that is still zero ends
+// the window; the walk resumes there (sleeping while that tile has not published).
+// Words below first_tile are never read: first_tile always publishes an inclusive value.
+constexpr int kLookbackWindow = 8;
+
+__device__ __forceinline__ uint32_t lookback_window(const uint32_t* lookback, uint32_t tile, uint32_t first_tile,
+                                                   uint32_t d)
+{
+  uint32_t excl = 0;
+  int64_t t     = int64_t(tile) - 1; // next tile to add
+  const int64_t lo = int64_t(first_tile);
+  while (t >= lo)
+  {
+    uint32_t v[kLookbackWindow];
+#pragma unroll
+    for (int j = 0; j < kLookbackWindow; ++j)
+    {
+      v[j] = (t - j >= lo) ? load_status(&lookback[uint64_t(t - j) * kRadix + d]) : kFlagGlobal;
+    }
+    bool done    = false;
+    int consumed = kLookbackWindow;
+#pragma unroll
+    for (int j = 0; j < kLookbackWindow; ++j)
+    {
+      if (!done && consumed == kLookbackWindow)
+      {
+        if (v[j] == 0u)
+        {
+          consumed = j; // not published yet: resume from this tile
+        }
+        else
+        {
+          excl += v[j] & kCountMask;
+          done = (v[j] & kFlagGlobal) != 0u;
+        }
+      }
+    }
+    if (done)
+    {
+      break;
+    }
+    t -= consumed;
+    if (consumed == 0)
+    {
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  return excl;
+}
+
+// Stores v at base[index] with the byte offset computed in 32 bits, so the store can use
+// the scalar base + 32-bit vector offset addressing form (arrays below 4 GiB only).
+template <class T>
+__device__ __forceinline__ void store_at(T* base, uint32_t index, const T& v)
+{
+  *reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(base) + index * uint32_t(sizeof(T))) = v;
+}
+
+template <class UK, class V, int THREADS, int IPT, bool SEG, bool FLOAT>
+__device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned char* smem)
 {
   using L              = OnesweepLayout<UK, V, THREADS, IPT>;
   constexpr int kWaves = L::kWaves;
@@ -207,27 +278,39 @@ __global__ void __launch_bounds__(THREADS) onesweep_kernel(OnesweepParams p)
   static_assert(THREADS >= kRadix, "one thread per digit is required");
   using VS = typename std::conditional<kHasV, V, uint8_t>::type;
 
-  extern __shared__ __align__(16) unsigned char smem[];
-  uint32_t* s_wcnt = reinterpret_cast<uint32_t*>(smem);
-  uint64_t* s_base = reinterpret_cast<uint64_t*>(smem + L::kOffBase);
-  uint32_t* s_misc = reinterpret_cast<uint32_t*>(smem + L::kOffMisc);
-  UK* s_keys       = reinterpret_cast<UK*>(smem + L::kOffKeys);
-  VS* s_vals       = reinterpret_cast<VS*>(smem + L::kOffVals);
+  uint32_t* s_wcnt   = reinterpret_cast<uint32_t*>(smem);
+  uint64_t* s_base   = reinterpret_cast<uint64_t*>(smem + L::kOffBase);
+  uint32_t* s_base32 = reinterpret_cast<uint32_t*>(smem + L::kOffBase); // low words of s_base
+  uint32_t* s_misc   = reinterpret_cast<uint32_t*>(smem + L::kOffMisc);
+  uint64_t* s_scan64 = reinterpret_cast<uint64_t*>(s_misc + 8);
+  UK* s_keys         = reinterpret_cast<UK*>(smem + L::kOffKeys);
+  VS* s_vals         = reinterpret_cast<VS*>(smem + L::kOffVals);
 
-  const uint32_t tid  = threadIdx.x;
-  const uint32_t lane = tid & 63u;
-  const uint32_t wave = tid >> 6;
+  const uint32_t tid   = threadIdx.x;
+  const uint32_t lane  = tid & 63u;
+  const uint32_t wave  = tid >> 6;
+  const uint32_t strip = wave * (IPT * 64) + lane;
+  uint32_t* wc         = s_wcnt + wave * kRadix;
+  const TwiddleT<UK, FLOAT> tw{UK(p.xor_mask)};
+  const uint32_t dmask = (1u << p.num_bits) - 1u;
+  const uint32_t limit = SEG ? *p.total_tiles : p.num_tiles;
+  UK* kout             = static_cast<UK*>(p.keys_out);
+  VS* vout             = static_cast<VS*>(p.vals_out);
 
-  for (uint32_t i = tid; i < uint32_t(kWaves * kRadix); i += THREADS)
-  {
-    s_wcnt[i] = 0;
-  }
+  // Tile ids follow the order in which workgroups start (an atomic counter, not
+  // blockIdx), so every tile a look-back waits on belongs to a workgroup that is already
+  // running.
   if (tid == 0)
   {
-    s_misc[0] = __hip_atomic_fetch_add(p.tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_misc[1] = __hip_atomic_fetch_add(p.tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  zero_wave_counters(wc);
   __syncthreads();
-  const uint32_t tile = s_misc[0];
+  const uint32_t tile = s_misc[1];
+  if (tile >= limit)
+  {
+    return; // block-uniform; only a segmented launch has spare workgroups
+  }
 
   // Clear this tile's entries of the other look-back buffer for the next launch; that
   // buffer is not read by this launch.
@@ -249,16 +332,12 @@ __global__ void __launch_bounds__(THREADS) onesweep_kernel(OnesweepParams p)
   uint64_t seg_begin  = 0;
   if constexpr (SEG)
   {
-    if (tile >= *p.total_tiles)
-    {
-      return; // block-uniform: the grid is an upper bound on the tile count
-    }
-    slot                  = p.tile_slot[tile];
-    first_tile            = p.slot_first_tile[slot];
-    seg_begin             = uint64_t(p.slot_range[2 * slot]);
+    slot                   = p.tile_slot[tile];
+    first_tile             = p.slot_first_tile[slot];
+    seg_begin              = uint64_t(p.slot_range[2 * slot]);
     const uint64_t seg_end = uint64_t(p.slot_range[2 * slot + 1]);
-    tile_base             = seg_begin + uint64_t(tile - first_tile) * kTile;
-    left                  = seg_end - tile_base;
+    tile_base              = seg_begin + uint64_t(tile - first_tile) * kTile;
+    left                   = seg_end - tile_base;
   }
   else
   {
@@ -266,36 +345,19 @@ __global__ void __launch_bounds__(THREADS) onesweep_kernel(OnesweepParams p)
     left      = p.num_items - tile_base;
   }
   const uint32_t valid = left < uint64_t(kTile) ? uint32_t(left) : uint32_t(kTile);
-  const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
-  const uint32_t dmask = (1u << p.num_bits) - 1u;
+  const bool full      = valid == uint32_t(kTile);
 
   // Load: wave w owns the strip [w*IPT*64, (w+1)*IPT*64) of the tile; item i of lane l
   // is strip element i*64 + l, so each load instruction reads 64 consecutive items.
-  const uint32_t strip = wave * (IPT * 64) + lane;
-  const UK* kin        = static_cast<const UK*>(p.keys_in) + tile_base + strip;
-  const VS* vin        = static_cast<const VS*>(p.vals_in) + tile_base + strip;
+  const UK* kin = static_cast<const UK*>(p.keys_in) + tile_base + strip;
+  const VS* vin = static_cast<const VS*>(p.vals_in) + tile_base + strip;
   UK keys[IPT];
-  VS vals[IPT];
-  uint32_t dig[IPT];
-  if (valid == uint32_t(kTile))
+  if (full)
   {
 #pragma unroll
     for (int i = 0; i < IPT; ++i)
     {
       keys[i] = kin[i * 64];
-    }
-    if constexpr (kHasV)
-    {
-#pragma unroll
-      for (int i = 0; i < IPT; ++i)
-      {
-        vals[i] = vin[i * 64];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < IPT; ++i)
-    {
-      dig[i] = extract_digit(tw(keys[i]), p.shift, dmask);
     }
   }
   else
@@ -303,88 +365,100 @@ __global__ void __launch_bounds__(THREADS) onesweep_kernel(OnesweepParams p)
 #pragma unroll
     for (int i = 0; i < IPT; ++i)
     {
-      const bool ok = strip + uint32_t(i * 64) < valid;
-      keys[i]       = ok ? kin[i * 64] : UK(0);
-      if constexpr (kHasV)
+      keys[i] = strip + uint32_t(i * 64) < valid ? kin[i * 64] : UK(0);
+    }
+  }
+
+  // Rank within the wave (the wave's own counters: no barrier needed around them).
+  // Items past the end take the largest digit; being last in input order they rank
+  // after every real item.
+  uint32_t rd[IPT]; // rank within the wave | digit << 16
+  wave_rank_packed<IPT>(rd, wc, [&](int i) {
+    const uint32_t d = extract_digit(tw(keys[i]), p.shift, dmask);
+    return (full || strip + uint32_t(i * 64) < valid) ? d : uint32_t(kRadix - 1);
+  });
+
+  // Values are read only now, so they are not live across the rank; their loads are in
+  // flight during the tile scan below.
+  VS vals[IPT];
+  if constexpr (kHasV)
+  {
+    if (full)
+    {
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
       {
-        if (ok)
+        vals[i] = vin[i * 64];
+      }
+    }
+    else
+    {
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        if (strip + uint32_t(i * 64) < valid)
         {
           vals[i] = vin[i * 64];
         }
       }
-      // items past the end take the largest digit; being last in input order they
-      // rank after every real item
-      dig[i] = ok ? extract_digit(tw(keys[i]), p.shift, dmask) : uint32_t(kRadix - 1);
     }
   }
-
-  // Rank within the wave.
-  uint32_t rank[IPT];
-  wave_rank<IPT>(dig, rank, s_wcnt + wave * kRadix);
   __syncthreads();
 
-  // Thread d (< kRadix) owns digit d: per-wave exclusive offsets and the tile count.
+  // Thread d (< kRadix) owns digit d: the tile count, published at once for the
+  // look-back of later tiles, then the exclusive scan over digits, then each wave's
+  // first slot for the digit.
+  uint32_t c[kWaves];
   uint32_t total = 0;
   if (tid < uint32_t(kRadix))
   {
 #pragma unroll
     for (int w = 0; w < kWaves; ++w)
     {
-      const uint32_t c         = s_wcnt[w * kRadix + tid];
-      s_wcnt[w * kRadix + tid] = total;
-      total += c;
+      c[w] = s_wcnt[w * kRadix + tid];
+      total += c[w];
     }
   }
   const uint32_t pad         = uint32_t(kTile) - valid;
   const uint32_t valid_total = (tid == uint32_t(kRadix - 1)) ? total - pad : total;
   if (tid < uint32_t(kRadix))
   {
-    store_status(&p.lookback[uint64_t(tile) * kRadix + tid],
-                 (tile == first_tile ? kFlagGlobal : kFlagPartial) | valid_total);
+    store_status(&p.lookback[uint64_t(tile) * kRadix + tid], (tile == first_tile ? kFlagGlobal : kFlagPartial) | valid_total);
   }
-
   uint32_t tile_items  = 0;
   const uint32_t start = radix_exclusive_scan<uint32_t>(total, s_misc + 4, tile_items);
-  uint64_t gstart      = 0;
+  uint64_t gstart      = 0; // thread d < kRadix: first output position of digit d
+  if constexpr (SEG)
   {
-    uint64_t* s_scan64 = reinterpret_cast<uint64_t*>(s_misc + 8);
+    const uint32_t* sb = p.seg_bins + (uint64_t(slot) * p.num_passes + p.pass) * kRadix;
     uint64_t all       = 0;
-    if constexpr (SEG)
-    {
-      const uint32_t* sb = p.seg_bins + (uint64_t(slot) * p.num_passes + p.pass) * kRadix;
-      gstart = seg_begin + radix_exclusive_scan<uint64_t>(tid < uint32_t(kRadix) ? uint64_t(sb[tid]) : 0, s_scan64, all);
-    }
-    else if (p.bins_are_starts)
-    {
-      gstart = tid < uint32_t(kRadix) ? p.bins[tid] : 0;
-    }
-    else
-    {
-      gstart = radix_exclusive_scan<uint64_t>(tid < uint32_t(kRadix) ? p.bins[tid] : 0, s_scan64, all);
-    }
+    gstart = seg_begin + radix_exclusive_scan<uint64_t>(tid < uint32_t(kRadix) ? uint64_t(sb[tid]) : 0, s_scan64, all);
+  }
+  else
+  {
+    gstart = tid < uint32_t(kRadix) ? p.bins[tid] : 0;
   }
   if (tid < uint32_t(kRadix))
   {
+    uint32_t run = start;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w)
     {
-      s_wcnt[w * kRadix + tid] += start;
+      s_wcnt[w * kRadix + tid] = run;
+      run += c[w];
     }
   }
   __syncthreads();
 
-  // Stage the tile in LDS in digit order.
-  {
-    const uint32_t* wc = s_wcnt + wave * kRadix;
+  // Stage keys and values in LDS in digit order.
 #pragma unroll
-    for (int i = 0; i < IPT; ++i)
+  for (int i = 0; i < IPT; ++i)
+  {
+    const uint32_t pos = wc[rd[i] >> 16] + (rd[i] & 0xffffu);
+    s_keys[pos]        = keys[i];
+    if constexpr (kHasV)
     {
-      const uint32_t pos = wc[dig[i]] + rank[i];
-      s_keys[pos]        = keys[i];
-      if constexpr (kHasV)
-      {
-        s_vals[pos] = vals[i];
-      }
+      s_vals[pos] = vals[i];
     }
   }
 
@@ -392,26 +466,9 @@ __global__ void __launch_bounds__(THREADS) onesweep_kernel(OnesweepParams p)
   // the inclusive prefix.
   if (tid < uint32_t(kRadix))
   {
-    uint32_t excl = 0;
+    const uint32_t excl = lookback_window(p.lookback, tile, first_tile, tid);
     if (tile > first_tile)
     {
-      int64_t t = int64_t(tile) - 1;
-      while (true)
-      {
-        const uint32_t* w = &p.lookback[uint64_t(t) * kRadix + tid];
-        uint32_t v        = load_status(w);
-        while (v == 0u)
-        {
-          __builtin_amdgcn_s_sleep(1);
-          v = load_status(w);
-        }
-        excl += v & kCountMask;
-        if (v & kFlagGlobal)
-        {
-          break;
-        }
-        --t;
-      }
       store_status(&p.lookback[uint64_t(tile) * kRadix + tid], kFlagGlobal | (excl + valid_total));
     }
     s_base[tid] = gstart + excl - start;
@@ -427,22 +484,76 @@ __global__ void __launch_bounds__(THREADS) onesweep_kernel(OnesweepParams p)
 
   // Write out in LDS order: consecutive threads take consecutive staged items, which
   // land at consecutive addresses within each digit's run.
-  UK* kout = static_cast<UK*>(p.keys_out);
-  VS* vout = static_cast<VS*>(p.vals_out);
-#pragma unroll
-  for (int i = 0; i < IPT; ++i)
+  if (full && p.small_offsets)
   {
-    const uint32_t pos = uint32_t(i) * THREADS + tid;
-    if (pos < valid)
+    // Full tile below 4 GiB: no bounds checks and 32-bit offsets (the low word of the
+    // 64-bit base is exact modulo 2^32, and base + pos fits in 32 bits).
+    constexpr int kChunk = (IPT % 4 == 0) ? 4 : 1;
+#pragma unroll
+    for (int c0 = 0; c0 < IPT; c0 += kChunk)
     {
-      const UK k         = s_keys[pos];
-      const uint32_t d   = extract_digit(tw(k), p.shift, dmask);
-      const uint64_t dst = s_base[d] + pos;
-      kout[dst]          = k;
+      UK k[kChunk];
+      uint32_t at[kChunk];
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j)
+      {
+        k[j] = s_keys[uint32_t(c0 + j) * THREADS + tid];
+      }
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j)
+      {
+        const uint32_t pos = uint32_t(c0 + j) * THREADS + tid;
+        at[j]              = s_base32[2 * extract_digit(tw(k[j]), p.shift, dmask)] + pos;
+        store_at(kout, at[j], k[j]);
+      }
       if constexpr (kHasV)
       {
-        vout[dst] = s_vals[pos];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j)
+        {
+          store_at(vout, at[j], s_vals[uint32_t(c0 + j) * THREADS + tid]);
+        }
       }
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      const uint32_t pos = uint32_t(i) * THREADS + tid;
+      if (pos < valid)
+      {
+        const UK k        = s_keys[pos];
+        const uint64_t at = s_base[extract_digit(tw(k), p.shift, dmask)] + pos;
+        kout[at]          = k;
+        if constexpr (kHasV)
+        {
+          vout[at] = s_vals[pos];
+        }
+      }
+    }
+  }
+}
+
+template <class UK, class V, int THREADS, int IPT, bool SEG>
+__global__ void __launch_bounds__(THREADS) onesweep_kernel(OnesweepParams p)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  if constexpr (sizeof(UK) == 1)
+  {
+    onesweep_tile<UK, V, THREADS, IPT, SEG, false>(p, smem); // no 8-bit floating point keys
+  }
+  else
+  {
+    // one block-uniform branch selects the key transform for the whole launch
+    if (p.float_mode)
+    {
+      onesweep_tile<UK, V, THREADS, IPT, SEG, true>(p, smem);
+    }
+    else
+    {
+      onesweep_tile<UK, V, THREADS, IPT, SEG, false>(p, smem);
     }
   }
 }
@@ -653,8 +764,15 @@ struct SegPassParams
 template <class UK, class V, int THREADS, int IPT>
 struct SegPassLayout
 {
+  // wave counters, tile bases, scan scratch, then keys and values side by side
   using Base                       = OnesweepLayout<UK, V, THREADS, IPT>;
-  static constexpr size_t kOffRun  = Base::kBytes + ((16 - Base::kBytes % 16) % 16);
+  static constexpr size_t kOffBase = Base::kOffBase;
+  static constexpr size_t kOffMisc = Base::kOffMisc;
+  static constexpr size_t kOffKeys = Base::kOffKeys;
+  static constexpr size_t kKeys    = size_t(Base::kTile) * sizeof(UK);
+  static constexpr size_t kVals    = Base::kHasV ? size_t(Base::kTile) * sizeof(V) : 0;
+  static constexpr size_t kOffVals = (kOffKeys + kKeys + 15) & ~size_t(15);
+  static constexpr size_t kOffRun  = (kOffVals + kVals + 15) & ~size_t(15);
   static constexpr size_t kBytes   = kOffRun + size_t(kRadix) * 8;
 };
 
@@ -670,11 +788,12 @@ __global__ void __launch_bounds__(THREADS) seg_pass_kernel(SegPassParams p)
 
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* s_wcnt = reinterpret_cast<uint32_t*>(smem);
-  uint64_t* s_base = reinterpret_cast<uint64_t*>(smem + L::kOffBase); // tile output base per digit
-  uint32_t* s_misc = reinterpret_cast<uint32_t*>(smem + L::kOffMisc);
-  UK* s_keys       = reinterpret_cast<UK*>(smem + L::kOffKeys);
-  VS* s_vals       = reinterpret_cast<VS*>(smem + L::kOffVals);
-  uint64_t* s_run  = reinterpret_cast<uint64_t*>(smem + SegPassLayout<UK, V, THREADS, IPT>::kOffRun);
+  using SL         = SegPassLayout<UK, V, THREADS, IPT>;
+  uint64_t* s_base = reinterpret_cast<uint64_t*>(smem + SL::kOffBase); // tile output base per digit
+  uint32_t* s_misc = reinterpret_cast<uint32_t*>(smem + SL::kOffMisc);
+  UK* s_keys       = reinterpret_cast<UK*>(smem + SL::kOffKeys);
+  VS* s_vals       = reinterpret_cast<VS*>(smem + SL::kOffVals);
+  uint64_t* s_run  = reinterpret_cast<uint64_t*>(smem + SL::kOffRun);
 
   const uint32_t tid   = threadIdx.x;
   const uint32_t lane  = tid & 63u;

@@ -143,7 +143,7 @@ CCCL_C_API int cccl_amd_graph_destroy(void* graph_exec);
  * bracketed by a pair of HIP events on its own stream.  read waits for the recorded
  * events, writes up to max (kind, milliseconds) pairs and returns how many there were.
  * Kinds: 0 init, 1 histogram, 2 onesweep pass, 3 one-workgroup sort, 4 segmented
- * binning, 5 segmented histogram, 6 segmented onesweep pass, 7 copy. */
+ * binning, 5 segmented histogram, 6 segmented onesweep pass, 7 copy, 8 digit-start scan. */
 CCCL_C_API void cccl_amd_profile_enable(int on);
 CCCL_C_API int cccl_amd_profile_read(int* kinds, float* ms, int max);
 
