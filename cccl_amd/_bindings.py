@@ -106,13 +106,13 @@ class radix_sort_config(ctypes.Structure):
         ("radix_bits", ctypes.c_int32),
         ("single_tile_threads", ctypes.c_int32),
         ("single_tile_ipt", ctypes.c_int32),
-        ("histogram_blocks_per_cu", ctypes.c_int32),
+        ("upsweep_blocks_per_cu", ctypes.c_int32),
         ("reserved", ctypes.c_int32 * 2),
     ]
 
     def as_tuple(self):
         return (self.onesweep_threads, self.onesweep_ipt, self.radix_bits, self.single_tile_threads,
-                self.single_tile_ipt, self.histogram_blocks_per_cu)
+                self.single_tile_ipt, self.upsweep_blocks_per_cu)
 
 
 class segmented_config(ctypes.Structure):

@@ -76,9 +76,10 @@ struct KernelHandle
   const char* role;
 };
 const KernelHandle g_single_tile{"single_tile"};
-const KernelHandle g_histogram{"histogram"};
+const KernelHandle g_upsweep{"upsweep"};
+const KernelHandle g_scan_bins{"scan_bins"};
+const KernelHandle g_downsweep{"downsweep"};
 const KernelHandle g_init{"init_bins_and_counters"};
-const KernelHandle g_onesweep{"onesweep"};
 
 char* dup_cstr(const std::string& s)
 {
@@ -189,15 +190,17 @@ CUresult compile_impl(
   r.runtime_policy_size = sizeof(cccl_amd_radix_sort_config_t);
   const std::string kv  = std::string(width_name(key_bytes)) + "," + width_name(int(vbytes));
   *name_slot(&r, 0)     = dup_cstr("cccl_amd::block_sort_kernel<" + kv + ">");
-  for (int i = 1; i <= 5; ++i)
+  *name_slot(&r, 1)     = dup_cstr(std::string("cccl_amd::upsweep_kernel<") + width_name(key_bytes) + ">");
+  *name_slot(&r, 2)     = dup_cstr("");
+  *name_slot(&r, 3)     = dup_cstr("cccl_amd::spine_kernel");
+  *name_slot(&r, 4)     = dup_cstr("cccl_amd::onesweep_kernel<" + kv + ">");
+  for (int i = 5; i <= 7; ++i)
   {
     *name_slot(&r, i) = dup_cstr("");
   }
-  *name_slot(&r, 6)  = dup_cstr(std::string("cccl_amd::histogram_kernel<") + width_name(key_bytes) + ">");
-  *name_slot(&r, 7)  = dup_cstr("");
   *name_slot(&r, 8)  = dup_cstr("cccl_amd::init_kernel");
   *name_slot(&r, 9)  = dup_cstr("");
-  *name_slot(&r, 10) = dup_cstr("cccl_amd::onesweep_kernel<" + kv + ">");
+  *name_slot(&r, 10) = dup_cstr("");
   *build             = r;
   return CUDA_SUCCESS;
 }
@@ -364,12 +367,13 @@ CCCL_C_API CUresult cccl_device_radix_sort_load(cccl_device_radix_sort_build_res
   }
   build->library = reinterpret_cast<CUlibrary>(const_cast<cccl_amd_library_st_impl*>(&g_library));
   build->single_tile_kernel            = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_single_tile));
-  build->histogram_kernel              = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_histogram));
+  build->upsweep_kernel                = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_upsweep));
+  build->scan_bins_kernel              = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_scan_bins));
+  build->downsweep_kernel              = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_downsweep));
   build->init_bins_and_counters_kernel = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_init));
-  build->onesweep_kernel               = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_onesweep));
-  build->upsweep_kernel = build->alt_upsweep_kernel = build->scan_bins_kernel = nullptr;
-  build->downsweep_kernel = build->alt_downsweep_kernel = nullptr;
-  build->exclusive_sum_kernel = build->init_lookback_kernel = nullptr;
+  build->alt_upsweep_kernel = build->alt_downsweep_kernel = nullptr;
+  build->histogram_kernel = build->exclusive_sum_kernel = nullptr;
+  build->init_lookback_kernel = build->onesweep_kernel = nullptr;
   return CUDA_SUCCESS;
 }
 
@@ -473,7 +477,7 @@ CCCL_C_API CUresult cccl_device_radix_sort(
     return CUDA_ERROR_INVALID_VALUE;
   }
   BuildPayload bp{};
-  if (!read_payload(build, bp) || build.runtime_policy == nullptr || build.onesweep_kernel == nullptr)
+  if (!read_payload(build, bp) || build.runtime_policy == nullptr || build.downsweep_kernel == nullptr)
   {
     return CUDA_ERROR_INVALID_VALUE;
   }

@@ -54,26 +54,40 @@ const KernelEntry* find_entry(const KernelEntry* list, size_t count, int key_byt
   return nullptr;
 }
 
-int device_cu_count()
+namespace {
+
+int cached_device_attribute(hipDeviceAttribute_t attr, int slot, int fallback)
 {
   static std::mutex mu;
-  static int cached[64] = {};
-  int dev               = 0;
+  static int cached[2][64] = {};
+  int dev                  = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
   {
-    return 256;
+    return fallback;
   }
   std::lock_guard<std::mutex> lock(mu);
-  if (cached[dev] == 0)
+  if (cached[slot][dev] == 0)
   {
     int n = 0;
-    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+    if (hipDeviceGetAttribute(&n, attr, dev) != hipSuccess || n <= 0)
     {
-      n = 256;
+      n = fallback;
     }
-    cached[dev] = n;
+    cached[slot][dev] = n;
   }
-  return cached[dev];
+  return cached[slot][dev];
+}
+
+} // namespace
+
+int device_cu_count()
+{
+  return cached_device_attribute(hipDeviceAttributeMultiprocessorCount, 0, 256);
+}
+
+int device_xcd_count()
+{
+  return cached_device_attribute(hipDeviceAttributeNumberOfXccs, 1, 1);
 }
 
 namespace {
@@ -292,14 +306,17 @@ void twiddle_constants(int key_bytes, int key_kind, bool descending, uint64_t& x
 cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64_t num_items)
 {
   cccl_amd_radix_sort_config_t c{};
-  c.radix_bits              = kRadixBits;
-  c.histogram_blocks_per_cu = 4;
-  c.single_tile_threads     = 256;
-  c.single_tile_ipt         = 16;
-  const int item_bytes      = key_bytes + val_bytes;
+  c.radix_bits            = kRadixBits;
+  c.upsweep_blocks_per_cu = 8;
+  c.single_tile_threads   = 256;
+  c.single_tile_ipt       = 16;
+  const int item_bytes    = key_bytes + val_bytes;
   if (num_items >= (uint64_t(1) << 22))
   {
-    c.onesweep_threads = item_bytes > 12 ? 256 : 512;
+    // Pairs of up to 8 bytes: the largest tile (longest digit runs per tile, so the
+    // scatter writes fewer partial lines); keys only: 512 x 16 (two workgroups per CU
+    // hide the rank's latency better).
+    c.onesweep_threads = item_bytes > 12 ? 256 : (val_bytes > 0 && item_bytes <= 8 ? 1024 : 512);
     c.onesweep_ipt     = 16;
   }
   else
@@ -431,31 +448,45 @@ hipError_t radix_sort(
     return hipSuccess;
   }
 
-  // Onesweep (dispatch_radix_sort.cuh:1698-1948).
+  // Digit passes (InvokePasses, dispatch_radix_sort.cuh:832-990): each pass is an
+  // upsweep (per-tile digit counts), a spine (scan of the counts) and a downsweep (rank
+  // and scatter), so no tile of a pass waits on another.
   const KernelEntry* os =
     find_entry(tab->onesweep, tab->num_onesweep, kb, vb, cfg.onesweep_threads, cfg.onesweep_ipt);
-  if (os == nullptr || cfg.radix_bits != kRadixBits)
+  const KernelEntry* up = find_entry(tab->upsweep, tab->num_upsweep, kb, 0, cfg.onesweep_threads, cfg.onesweep_ipt);
+  if (os == nullptr || up == nullptr || cfg.radix_bits != kRadixBits)
   {
     return hipErrorInvalidValue;
   }
-  const uint64_t tile         = uint64_t(os->threads) * uint64_t(os->ipt);
-  const uint64_t portion_size = (kMaxPortionItems / tile) * tile;
-  const int passes            = ceil_div(pr.end_bit - pr.begin_bit, kRadixBits);
-  const uint64_t portions     = ceil_div(n, portion_size);
-  const uint64_t max_tiles    = ceil_div(std::min(n, portion_size), tile);
-  const bool need_tmp         = !pr.overwrite && passes > 1;
+  const uint64_t tile  = uint64_t(os->threads) * uint64_t(os->ipt);
+  const int passes     = ceil_div(pr.end_bit - pr.begin_bit, kRadixBits);
+  const uint64_t tiles = ceil_div(n, tile);
+  if (tiles > 0x7fffffffull)
+  {
+    set_last_error("%llu items need more than 2^31 tiles of %llu", static_cast<unsigned long long>(n),
+                   static_cast<unsigned long long>(tile));
+    return hipErrorInvalidValue;
+  }
+  // Blocks of consecutive tiles for the upsweep: about upsweep_blocks_per_cu per compute
+  // unit, each holding fewer than 2^32 items (block sums are 32-bit).
+  const uint64_t want_blocks = uint64_t(device_cu_count()) * uint64_t(std::max(1, cfg.upsweep_blocks_per_cu));
+  const uint64_t max_tpb     = (0xffffffffull / tile);
+  const uint64_t tpb         = std::max<uint64_t>(1, std::min(ceil_div(tiles, want_blocks), max_tpb));
+  const uint64_t blocks      = ceil_div(tiles, tpb);
+  const bool need_tmp        = !pr.overwrite && passes > 1;
 
-  size_t sizes[7] = {
-    size_t(passes) * kRadix * sizeof(uint64_t), // digit counts of every pass
-    size_t(portions) * passes * kRadix * sizeof(uint64_t), // digit starts of every portion (portions > 0)
-    size_t(max_tiles) * kRadix * sizeof(uint32_t), // look-back buffer A
-    size_t(max_tiles) * kRadix * sizeof(uint32_t), // look-back buffer B
+  size_t sizes[8] = {
+    size_t(tiles) * kRadix * sizeof(uint32_t), // tile_off
+    size_t(blocks) * kRadix * sizeof(uint32_t), // block_sum
+    size_t(blocks) * kRadix * sizeof(uint64_t), // block_prefix
+    size_t(kRadix) * sizeof(uint64_t), // totals
+    size_t(kRadix) * sizeof(uint64_t), // digit_start
+    size_t(passes) * sizeof(uint32_t), // spine completion counters
     need_tmp ? size_t(n) * kb : 0, // extra key buffer
     need_tmp && !keys_only ? size_t(n) * vb : 0, // extra value buffer
-    size_t(portions) * passes * sizeof(uint32_t), // tile counters
   };
-  void* parts[7] = {};
-  if (hipError_t e = alias_temporaries(d_temp_storage, temp_storage_bytes, parts, sizes, 7); e != hipSuccess)
+  void* parts[8] = {};
+  if (hipError_t e = alias_temporaries(d_temp_storage, temp_storage_bytes, parts, sizes, 8); e != hipSuccess)
   {
     return e;
   }
@@ -463,35 +494,18 @@ hipError_t radix_sort(
   {
     return hipSuccess;
   }
-  uint64_t* d_bins     = static_cast<uint64_t*>(parts[0]);
-  uint64_t* d_starts   = static_cast<uint64_t*>(parts[1]);
-  uint32_t* d_lb[2]    = {static_cast<uint32_t*>(parts[2]), static_cast<uint32_t*>(parts[3])};
-  void* d_keys_tmp2    = parts[4];
-  void* d_vals_tmp2    = parts[5];
-  uint32_t* d_counters = static_cast<uint32_t*>(parts[6]);
+  uint32_t* d_tile_off     = static_cast<uint32_t*>(parts[0]);
+  uint32_t* d_block_sum    = static_cast<uint32_t*>(parts[1]);
+  uint64_t* d_block_prefix = static_cast<uint64_t*>(parts[2]);
+  uint64_t* d_totals       = static_cast<uint64_t*>(parts[3]);
+  uint64_t* d_digit_start  = static_cast<uint64_t*>(parts[4]);
+  uint32_t* d_done         = static_cast<uint32_t*>(parts[5]);
+  void* d_keys_tmp2        = parts[6];
+  void* d_vals_tmp2        = parts[7];
 
   {
-    InitParams ip{d_bins, uint64_t(passes) * kRadix, d_counters, portions * passes, d_lb[0], max_tiles * kRadix};
-    const uint64_t most = std::max<uint64_t>(ip.num_lookback, ip.num_bins);
-    const uint64_t grid = std::min<uint64_t>(ceil_div<uint64_t>(most, 256), 2048);
-    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&init_kernel), grid, 256, 0, stream, &ip, 0);
-        e != hipSuccess)
-    {
-      return e;
-    }
-  }
-  {
-    HistogramParams hp{keys.current(), n, d_bins, xor_mask, float_mode, pr.begin_bit, pr.end_bit, passes};
-    const uint64_t per_block = uint64_t(kHistThreads) * kHistUnroll;
-    const uint64_t cap       = uint64_t(device_cu_count()) * uint64_t(std::max(1, cfg.histogram_blocks_per_cu));
-    const uint64_t grid      = std::max<uint64_t>(1, std::min<uint64_t>(cap, ceil_div(n, per_block)));
-    if (hipError_t e = launch_kernel(tab->histogram, grid, kHistThreads, 0, stream, &hp, 1); e != hipSuccess)
-    {
-      return e;
-    }
-    void* args = d_bins;
-    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&digit_starts_kernel), uint64_t(passes), kRadix, 0,
-                                     stream, &args, 8);
+    InitParams ip{d_done, uint64_t(passes)};
+    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&init_kernel), 1, 256, 0, stream, &ip, 0);
         e != hipSuccess)
     {
       return e;
@@ -506,39 +520,57 @@ hipError_t radix_sort(
     vals.buf[1] = d_vals_tmp2;
   }
 
-  int launch_index = 0;
   for (int bit = pr.begin_bit, pass = 0; bit < pr.end_bit; bit += kRadixBits, ++pass)
   {
     const int num_bits = std::min(pr.end_bit - bit, kRadixBits);
-    for (uint64_t portion = 0; portion < portions; ++portion)
+
+    UpsweepParams upp{};
+    upp.keys            = keys.current();
+    upp.tile_off        = d_tile_off;
+    upp.block_sum       = d_block_sum;
+    upp.num_items       = n;
+    upp.xor_mask        = xor_mask;
+    upp.num_tiles       = uint32_t(tiles);
+    upp.tiles_per_block = uint32_t(tpb);
+    upp.num_blocks      = uint32_t(blocks);
+    upp.float_mode      = float_mode;
+    upp.shift           = bit;
+    upp.num_bits        = num_bits;
+    if (hipError_t e = launch_kernel(up->fn, blocks, up->threads, 0, stream, &upp, 1); e != hipSuccess)
     {
-      const uint64_t first = portion * portion_size;
-      const uint64_t items = std::min(n - first, portion_size);
-      const uint64_t tiles = ceil_div(items, tile);
-      OnesweepParams op{};
-      op.keys_in         = static_cast<const char*>(keys.current()) + first * kb;
-      op.keys_out        = keys.alternate();
-      op.vals_in         = keys_only ? nullptr : static_cast<const char*>(vals.current()) + first * vb;
-      op.vals_out        = keys_only ? nullptr : vals.alternate();
-      op.lookback        = d_lb[launch_index & 1];
-      op.lookback_next   = d_lb[(launch_index & 1) ^ 1];
-      op.tile_counter    = d_counters + portion * passes + pass;
-      op.bins            = portion == 0 ? d_bins + size_t(pass) * kRadix : d_starts + (portion * passes + pass) * kRadix;
-      op.bins_next       = portion + 1 < portions ? d_starts + ((portion + 1) * passes + pass) * kRadix : nullptr;
-      op.num_items       = items;
-      op.xor_mask        = xor_mask;
-      op.num_tiles       = uint32_t(tiles);
-      op.zero_tiles      = uint32_t(max_tiles);
-      op.float_mode      = float_mode;
-      op.shift           = bit;
-      op.num_bits        = num_bits;
-      op.small_offsets   = n * uint64_t(std::max(kb, vb)) < (uint64_t(1) << 32) ? 1 : 0;
-      if (hipError_t e = launch_kernel(os->fn, tiles, os->threads, os->lds_bytes, stream, &op, 2); e != hipSuccess)
-      {
-        return e;
-      }
-      ++launch_index;
+      return e;
     }
+
+    SpineParams sp{d_block_sum, d_block_prefix, d_totals, d_digit_start, d_done + pass, uint32_t(blocks)};
+    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&spine_kernel), kRadix, kRadix, 0, stream, &sp, 8);
+        e != hipSuccess)
+    {
+      return e;
+    }
+
+    OnesweepParams op{};
+    op.keys_in         = keys.current();
+    op.keys_out        = keys.alternate();
+    op.vals_in         = keys_only ? nullptr : vals.current();
+    op.vals_out        = keys_only ? nullptr : vals.alternate();
+    op.num_items       = n;
+    op.xor_mask        = xor_mask;
+    op.num_tiles       = uint32_t(tiles);
+    op.float_mode      = float_mode;
+    op.shift           = bit;
+    op.num_bits        = num_bits;
+    op.small_offsets   = n * uint64_t(std::max(kb, vb)) < (uint64_t(1) << 32) ? 1 : 0;
+    op.tile_off        = d_tile_off;
+    op.block_prefix    = d_block_prefix;
+    op.digit_start     = d_digit_start;
+    op.tiles_per_block = uint32_t(tpb);
+    op.num_blocks      = uint32_t(blocks);
+    op.num_xcds        = uint32_t(device_xcd_count());
+    if (hipError_t e = launch_kernel(os->fn, tiles, os->threads, os->lds_bytes, stream, &op, 2); e != hipSuccess)
+    {
+      return e;
+    }
+
     if (!pr.overwrite && pass == 0)
     {
       keys = passes % 2 == 0 ? DoubleBufferRef{{d_keys_tmp, d_keys_tmp2}, 0} : DoubleBufferRef{{d_keys_tmp2, d_keys_tmp}, 0};

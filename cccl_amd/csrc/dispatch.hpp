@@ -84,6 +84,9 @@ const KernelEntry* find_entry(const KernelEntry* list, size_t count, int key_byt
 
 int device_cu_count();
 
+// Number of XCDs (accelerator complex dies, each with its own L2) of the current device.
+int device_xcd_count();
+
 // Launches fn<<<grid, threads, lds, stream>>>(*params); kind tags the timing record.
 hipError_t launch_kernel(const void* fn, uint64_t grid, int threads, size_t lds, hipStream_t stream, void* params, int kind);
 

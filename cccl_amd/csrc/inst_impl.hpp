@@ -37,6 +37,12 @@ KernelEntry onesweep_entry()
   }
 }
 
+template <class UK, int T, int I>
+KernelEntry upsweep_entry()
+{
+  return {int(sizeof(UK)), 0, T, I, reinterpret_cast<const void*>(&upsweep_kernel<UK, T, I>), 0};
+}
+
 template <class UK, class V, int T, int I>
 KernelEntry seg_onesweep_entry()
 {
@@ -87,6 +93,11 @@ KernelEntry seg_pass_entry()
 template <class UK>
 KeyWidthTable make_table()
 {
+  static const KernelEntry upsweep[] = {
+#define CCCL_AMD_X(T, I) upsweep_entry<UK, T, I>(),
+    CCCL_AMD_ONESWEEP_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
   static const KernelEntry onesweep[] = {
 #define CCCL_AMD_X(T, I) CCCL_AMD_ALL_VALUES(onesweep_entry, UK, T, I)
     CCCL_AMD_ONESWEEP_SHAPES(CCCL_AMD_X)
@@ -107,7 +118,8 @@ KeyWidthTable make_table()
     CCCL_AMD_SEG_ONESWEEP_SHAPES(CCCL_AMD_X)
 #undef CCCL_AMD_X
   };
-  return {reinterpret_cast<const void*>(&histogram_kernel<UK>),
+  return {upsweep,
+          sizeof(upsweep) / sizeof(upsweep[0]),
           onesweep,
           sizeof(onesweep) / sizeof(onesweep[0]),
           block_sort,

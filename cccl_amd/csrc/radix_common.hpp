@@ -164,10 +164,22 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan<uint32_t>(uint32_t v)
   return uint32_t(x);
 }
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also waits for
+// every outstanding global load and store of the wave (it is a fence for all address
+// spaces), which would stall loads issued ahead of the barrier to hide their latency.
+// Only for barriers that protect LDS data: global memory written by one thread and read
+// by another across it is not ordered.
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Exclusive scan of one value per thread over the first kRadix threads of a block
 // (threads >= kRadix contribute nothing).  s_wave must hold kRadix/64 entries.  Must
-// be reached by every thread of the block.
-template <class T>
+// be reached by every thread of the block.  LDS_ONLY uses lds_barrier().
+template <class T, bool LDS_ONLY = false>
 __device__ __forceinline__ T radix_exclusive_scan(T v, T* s_wave, T& total)
 {
   const uint32_t tid  = threadIdx.x;
@@ -181,7 +193,14 @@ __device__ __forceinline__ T radix_exclusive_scan(T v, T* s_wave, T& total)
       s_wave[wave] = inc;
     }
   }
-  __syncthreads();
+  if constexpr (LDS_ONLY)
+  {
+    lds_barrier();
+  }
+  else
+  {
+    __syncthreads();
+  }
   T before = T(0);
   T all    = T(0);
 #pragma unroll
@@ -195,7 +214,14 @@ __device__ __forceinline__ T radix_exclusive_scan(T v, T* s_wave, T& total)
     all += s;
   }
   total = all;
-  __syncthreads();
+  if constexpr (LDS_ONLY)
+  {
+    lds_barrier();
+  }
+  else
+  {
+    __syncthreads();
+  }
   return before + inc - v;
 }
 

@@ -1,7 +1,7 @@
 // Table of precompiled gfx950 kernel instantiations.
 //
 // Each key width has its own translation unit (inst_k1.hip ... inst_k8.hip) that
-// instantiates the histogram, onesweep and block-sort kernels for every value width
+// instantiates the upsweep, onesweep and block-sort kernels for every value width
 // and every launch shape listed below, and returns them as a flat table.  The host
 // dispatch picks an entry by (key bytes, value bytes, threads, items per thread).
 #pragma once
@@ -26,7 +26,8 @@ struct KernelEntry
 
 struct KeyWidthTable
 {
-  const void* histogram;
+  const KernelEntry* upsweep; // val_bytes 0; one per onesweep shape
+  size_t num_upsweep;
   const KernelEntry* onesweep;
   size_t num_onesweep;
   const KernelEntry* block_sort;
@@ -53,7 +54,8 @@ const KeyWidthTable* kernels_for_key_bytes(int key_bytes);
   X(512, 8)                         \
   X(512, 12)                        \
   X(512, 16)                        \
-  X(1024, 8)
+  X(1024, 8)                        \
+  X(1024, 16)
 
 // Launch shapes compiled for the batched onesweep of large segments.
 #define CCCL_AMD_SEG_ONESWEEP_SHAPES(X) \

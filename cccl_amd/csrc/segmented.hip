@@ -172,7 +172,9 @@ hipError_t segmented_radix_sort(
   {
     return hipErrorInvalidValue;
   }
-  if (cfg.strategy == 1)
+  // The batched onesweep's look-back status words hold 30-bit counts, so a segment may
+  // hold at most kMaxPortionItems items there; larger inputs take the per-segment passes.
+  if (cfg.strategy == 1 || n > kMaxPortionItems)
   {
     return segmented_per_segment_passes(
       d_temp_storage, temp_storage_bytes, keys, vals, pr, num_segments, d_begin_offsets, d_end_offsets, offset_bytes, cfg,

@@ -1,18 +1,20 @@
 // gfx950 kernels of the device-wide LSD radix sort (wave64).
 //
-//   init_kernel       zeroes digit bins, tile counters and the first look-back buffer
-//                     (role of DeviceRadixSortInitKernel, kernel_radix_sort.cuh:475-496)
-//   histogram_kernel  one read of the keys builds the digit histogram of every pass
-//                     (role of DeviceRadixSortHistogramKernel, kernel_radix_sort.cuh:447-473,
-//                     agent_radix_sort_histogram.cuh:205-246)
-//   digit_starts_kernel exclusive sum of each pass's digit counts (role of
-//                     DeviceRadixSortExclusiveSumKernel, kernel_radix_sort.cuh:563-606)
-//   onesweep_kernel   one digit pass: wave-ballot rank, decoupled look-back over tiles,
-//                     LDS-staged scatter (role of DeviceRadixSortOnesweepKernel,
-//                     kernel_radix_sort.cuh:498-558, agent_radix_sort_onesweep.cuh:667-697)
+//   init_kernel       zeroes the per-pass spine counters
+//                     (role of DeviceRadixSortInitKernel, kernel_radix_sort.cuh:476-496)
+//   upsweep_kernel    digit counts of every tile of one pass (DeviceRadixSortUpsweepKernel,
+//                     kernel_radix_sort.cuh:85-129)
+//   spine_kernel      per-digit exclusive scan of the block counts and the digit starts
+//                     (RadixSortScanBinsKernel, kernel_radix_sort.cuh:147-186)
+//   onesweep_kernel   one digit pass over tiles: wave-ballot rank, LDS-staged scatter.
+//                     SEG == false is the plain sort's downsweep (DeviceRadixSortDownsweepKernel,
+//                     kernel_radix_sort.cuh:244-291) reading its digit starts from the spine;
+//                     SEG == true is the segmented sort's onesweep with decoupled look-back
+//                     (DeviceRadixSortOnesweepKernel, kernel_radix_sort.cuh:507-558,
+//                     agent_radix_sort_onesweep.cuh:667-697)
 //   block_sort_kernel whole LSD sort of up to one tile inside one workgroup, one
 //                     workgroup per segment (role of DeviceRadixSortSingleTileKernel,
-//                     kernel_radix_sort.cuh:330-434, and of the per-segment kernel
+//                     kernel_radix_sort.cuh:337-434, and of the per-segment kernel
 //                     kernel_segmented_radix_sort.cuh:100-291 for segments that fit)
 #pragma once
 
@@ -24,164 +26,327 @@ namespace cccl_amd {
 
 struct InitParams
 {
-  uint64_t* bins;
-  uint64_t num_bins;
   uint32_t* counters;
   uint64_t num_counters;
-  uint32_t* lookback;
-  uint64_t num_lookback;
 };
 
 static __global__ void __launch_bounds__(256) init_kernel(InitParams p)
 {
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < p.num_bins; i += stride)
-  {
-    p.bins[i] = 0;
-  }
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < p.num_counters; i += stride)
   {
     p.counters[i] = 0;
   }
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < p.num_lookback; i += stride)
-  {
-    p.lookback[i] = 0;
-  }
 }
 
-// Turns the digit counts of every pass (bins[pass][kRadix], one workgroup per pass) into
-// exclusive starts in place, so each onesweep tile reads its digit starts directly.
-static __global__ void __launch_bounds__(kRadix) digit_starts_kernel(uint64_t* bins)
-{
-  __shared__ uint64_t s_wave[kRadix / 64];
-  uint64_t* b        = bins + uint64_t(blockIdx.x) * kRadix;
-  uint64_t all       = 0;
-  const uint64_t ex  = radix_exclusive_scan<uint64_t>(b[threadIdx.x], s_wave, all);
-  b[threadIdx.x]     = ex;
-}
+// ---------------------------------------------------------------------------
+// Plain sort, one digit pass = upsweep + spine + downsweep
+// ---------------------------------------------------------------------------
+// Tiles of THREADS*IPT consecutive items are grouped into blocks of tiles_per_block
+// tiles.  The upsweep counts the digits of every tile; the spine scans the block sums of
+// each digit; the downsweep then knows where every tile's digit runs start without
+// waiting on any other tile:
+//   start of digit d in tile t = digit_start[d] + block_prefix[d][t / G] + tile_off[t][d]
+// This is the reference's three-kernel pass (InvokePass, dispatch_radix_sort.cuh:328-444;
+// DeviceRadixSortUpsweepKernel, RadixSortScanBinsKernel and DeviceRadixSortDownsweepKernel,
+// kernel_radix_sort.cuh:85-291) rather than its onesweep: with the spine per pass there is
+// no cross-tile wait, which on this GPU (look-back round trips of microseconds between
+// XCDs) costs less than the extra read of the keys.
 
-struct HistogramParams
-{
-  const void* keys;
-  uint64_t num_items;
-  uint64_t* bins; // [passes][kRadix]
-  uint64_t xor_mask;
-  int float_mode;
-  int begin_bit;
-  int end_bit;
-  int num_passes;
-};
-
-// Sub-histogram copies per block: lanes l and l' with l % kHistParts != l' % kHistParts
-// never touch the same LDS word, which divides same-address atomic conflicts when the
-// key entropy is low.
+// Histogram sub-copies of the segmented histogram: lanes l and l' with
+// l % kHistParts != l' % kHistParts never touch the same LDS word, which divides
+// same-address atomic conflicts when the key entropy is low.
 constexpr int kHistParts   = 4;
 constexpr int kHistThreads = 256;
-constexpr int kHistUnroll  = 8;
 
-template <class UK>
-__global__ void __launch_bounds__(kHistThreads) histogram_kernel(HistogramParams p)
+struct UpsweepParams
 {
-  constexpr int kMaxPasses = int(sizeof(UK) * 8 / kRadixBits);
-  __shared__ uint32_t s_hist[kMaxPasses * kRadix * kHistParts];
+  const void* keys;
+  uint32_t* tile_off; // [num_tiles][kRadix]: digit counts of the earlier tiles of the block
+  uint32_t* block_sum; // [kRadix][num_blocks]: digit counts of each block
+  uint64_t num_items;
+  uint64_t xor_mask;
+  uint32_t num_tiles;
+  uint32_t tiles_per_block;
+  uint32_t num_blocks;
+  int float_mode;
+  int shift;
+  int num_bits;
+};
 
-  for (int i = threadIdx.x; i < kMaxPasses * kRadix * kHistParts; i += kHistThreads)
-  {
-    s_hist[i] = 0;
-  }
-  __syncthreads();
+// One workgroup per block of tiles.  Each wave counts into its own LDS counters; the
+// running sums at each tile boundary are that tile's tile_off row.  A wave whose 64
+// digits are all equal (low-entropy keys) adds them with one atomic instead of 64
+// conflicting ones.
+template <class UK, int THREADS, int IPT, bool FLOAT>
+__device__ __forceinline__ void upsweep_block(const UpsweepParams& p, uint32_t* s_cnt)
+{
+  constexpr int kWaves = THREADS / 64;
+  constexpr int kTile  = THREADS * IPT;
+  const uint32_t tid   = threadIdx.x;
+  uint32_t* wc         = s_cnt + (tid >> 6) * kRadix;
+  const TwiddleT<UK, FLOAT> tw{UK(p.xor_mask)};
+  const uint32_t dmask = (1u << p.num_bits) - 1u;
+  const UK* keys       = static_cast<const UK*>(p.keys);
+  const uint32_t b     = blockIdx.x;
+  const uint32_t t0    = b * p.tiles_per_block;
+  const uint32_t t1    = min(t0 + p.tiles_per_block, p.num_tiles);
 
-  const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
-  const UK* keys      = static_cast<const UK*>(p.keys);
-  const uint32_t part = threadIdx.x % kHistParts;
-  const int passes    = p.num_passes;
-
+  zero_wave_counters(wc);
   auto count = [&](UK k) {
-    const UK t = tw(k);
-    for (int pass = 0; pass < passes; ++pass)
+    const uint32_t d  = extract_digit(tw(k), p.shift, dmask);
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const uint64_t eq = __ballot(d == d0);
+    if (eq == __builtin_amdgcn_read_exec()) // wave-uniform
     {
-      const int shift     = p.begin_bit + pass * kRadixBits;
-      const int bits      = min(kRadixBits, p.end_bit - shift);
-      const uint32_t mask = (1u << bits) - 1u;
-      const uint32_t d    = extract_digit(t, shift, mask);
-      __hip_atomic_fetch_add(
-        &s_hist[(pass * kRadix + d) * kHistParts + part], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (mask_prefix_count(eq) == 0u)
+      {
+        __hip_atomic_fetch_add(wc + d0, uint32_t(__builtin_popcountll(eq)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+    else
+    {
+      __hip_atomic_fetch_add(wc + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   };
 
-  const uint64_t n          = p.num_items;
-  const uint64_t block_span = uint64_t(kHistThreads) * kHistUnroll;
-  const uint64_t grid_span  = block_span * gridDim.x;
-  uint64_t base             = uint64_t(blockIdx.x) * block_span;
-  for (; base + block_span <= n; base += grid_span)
-  {
-    UK k[kHistUnroll];
-#pragma unroll
-    for (int u = 0; u < kHistUnroll; ++u)
+  // Row t of tile_off: the running counts before tile t.
+  auto tile_start = [&](uint32_t t) {
+    lds_barrier(); // the counts of the block's earlier tiles are complete
+    if (tid < uint32_t(kRadix))
     {
-      k[u] = keys[base + uint64_t(u) * kHistThreads + threadIdx.x];
+      uint32_t s = 0;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w)
+      {
+        s += s_cnt[w * kRadix + tid];
+      }
+      p.tile_off[uint64_t(t) * kRadix + tid] = s;
     }
-#pragma unroll
-    for (int u = 0; u < kHistUnroll; ++u)
+    lds_barrier(); // read before any of this tile's counts are added
+  };
+
+  // Full tiles: tile t + 1 is loaded while tile t is counted, from two register sets
+  // used alternately (no copies between them, so the loads stay in flight).  Counting
+  // does not care which thread holds which key, so a 16-byte aligned array is read with
+  // 16-byte loads.
+  const uint32_t full_end = (t1 == p.num_tiles && p.num_items % kTile != 0) ? t1 - 1 : t1;
+  auto stream_full = [&](auto loader) {
+    UK ka[IPT];
+    UK kb[IPT];
+    uint32_t t = t0;
+    if (t < full_end)
     {
-      count(k[u]);
+      loader(t, ka);
+    }
+    while (t < full_end)
+    {
+      if (t + 1 < full_end)
+      {
+        loader(t + 1, kb);
+      }
+      tile_start(t);
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        count(ka[i]);
+      }
+      if (++t >= full_end)
+      {
+        break;
+      }
+      if (t + 1 < full_end)
+      {
+        loader(t + 1, ka);
+      }
+      tile_start(t);
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        count(kb[i]);
+      }
+      ++t;
+    }
+  };
+  bool vector_path = false;
+  if constexpr ((IPT * sizeof(UK)) % 16 == 0)
+  {
+    constexpr int kVec = int(IPT * sizeof(UK) / 16);
+    constexpr int kPer = int(16 / sizeof(UK));
+    if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0u)
+    {
+      vector_path = true;
+      stream_full([&](uint32_t t, UK(&k)[IPT]) {
+        const uint4* src = reinterpret_cast<const uint4*>(keys + uint64_t(t) * kTile);
+#pragma unroll
+        for (int j = 0; j < kVec; ++j)
+        {
+          const uint4 v = src[j * THREADS + tid];
+          UK e[kPer];
+          __builtin_memcpy(e, &v, 16);
+#pragma unroll
+          for (int q = 0; q < kPer; ++q)
+          {
+            k[j * kPer + q] = e[q];
+          }
+        }
+      });
     }
   }
-  if (base < n)
+  if (!vector_path)
   {
+    stream_full([&](uint32_t t, UK(&k)[IPT]) {
+      const UK* src = keys + uint64_t(t) * kTile;
 #pragma unroll
-    for (int u = 0; u < kHistUnroll; ++u)
-    {
-      const uint64_t i = base + uint64_t(u) * kHistThreads + threadIdx.x;
-      if (i < n)
+      for (int i = 0; i < IPT; ++i)
       {
-        count(keys[i]);
+        k[i] = src[i * THREADS + tid];
+      }
+    });
+  }
+  // The partial last tile of the array, item by item (item i*THREADS + tid).
+  if (full_end < t1)
+  {
+    const uint64_t base  = uint64_t(full_end) * kTile;
+    const uint32_t valid = uint32_t(p.num_items - base);
+    UK k[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      const uint32_t pos = uint32_t(i) * THREADS + tid;
+      k[i]               = pos < valid ? keys[base + pos] : UK(0);
+    }
+    tile_start(full_end);
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      if (uint32_t(i) * THREADS + tid < valid)
+      {
+        count(k[i]);
       }
     }
   }
-  __syncthreads();
-
-  for (int i = threadIdx.x; i < passes * kRadix; i += kHistThreads)
+  lds_barrier();
+  if (tid < uint32_t(kRadix))
   {
     uint32_t s = 0;
 #pragma unroll
-    for (int q = 0; q < kHistParts; ++q)
+    for (int w = 0; w < kWaves; ++w)
     {
-      s += s_hist[i * kHistParts + q];
+      s += s_cnt[w * kRadix + tid];
     }
-    if (s != 0)
+    p.block_sum[uint64_t(tid) * p.num_blocks + b] = s;
+  }
+}
+
+template <class UK, int THREADS, int IPT>
+__global__ void __launch_bounds__(THREADS) upsweep_kernel(UpsweepParams p)
+{
+  __shared__ uint32_t s_cnt[(THREADS / 64) * kRadix];
+  if constexpr (sizeof(UK) == 1)
+  {
+    upsweep_block<UK, THREADS, IPT, false>(p, s_cnt);
+  }
+  else
+  {
+    if (p.float_mode)
     {
-      __hip_atomic_fetch_add(&p.bins[i], uint64_t(s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      upsweep_block<UK, THREADS, IPT, true>(p, s_cnt);
+    }
+    else
+    {
+      upsweep_block<UK, THREADS, IPT, false>(p, s_cnt);
     }
   }
 }
 
-// One onesweep launch covers either one portion of a plain sort (SEG == false: tile t
-// holds items [t*TILE, (t+1)*TILE) of the portion) or every large segment of a batched
-// segmented sort at once (SEG == true: tiles are grouped per segment, the tile range of
-// segment slot s starts at slot_first_tile[s], and look-back stops at the segment's
-// first tile).
+struct SpineParams
+{
+  const uint32_t* block_sum; // [kRadix][num_blocks]
+  uint64_t* block_prefix; // [kRadix][num_blocks]: exclusive prefix over blocks
+  uint64_t* totals; // [kRadix]
+  uint64_t* digit_start; // [kRadix]: exclusive prefix of totals over digits
+  uint32_t* done; // zero on entry
+  uint32_t num_blocks;
+};
+
+// One workgroup per digit scans that digit's block sums.  The workgroup that finishes
+// last (an acquire-release counter orders every other workgroup's total before it) also
+// scans the digit totals into digit starts.
+static __global__ void __launch_bounds__(kRadix) spine_kernel(SpineParams p)
+{
+  __shared__ uint64_t s_wave[kRadix / 64];
+  __shared__ uint32_t s_last;
+  const uint32_t d     = blockIdx.x;
+  const uint32_t tid   = threadIdx.x;
+  const uint32_t nb    = p.num_blocks;
+  const uint32_t per   = (nb + kRadix - 1) / kRadix;
+  const uint32_t lo    = min(tid * per, nb);
+  const uint32_t hi    = min(lo + per, nb);
+  const uint32_t* in   = p.block_sum + uint64_t(d) * nb;
+  uint64_t* out        = p.block_prefix + uint64_t(d) * nb;
+  uint64_t sum         = 0;
+  for (uint32_t j = lo; j < hi; ++j)
+  {
+    sum += in[j];
+  }
+  uint64_t all = 0;
+  uint64_t run = radix_exclusive_scan<uint64_t>(sum, s_wave, all);
+  for (uint32_t j = lo; j < hi; ++j)
+  {
+    const uint32_t c = in[j];
+    out[j]           = run;
+    run += c;
+  }
+  if (tid == 0)
+  {
+    p.totals[d]         = all;
+    const uint32_t prev = __hip_atomic_fetch_add(p.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last              = prev == uint32_t(kRadix - 1) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (s_last != 0u) // block-uniform
+  {
+    const uint64_t v = __hip_atomic_load(&p.totals[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t grand   = 0;
+    p.digit_start[tid] = radix_exclusive_scan<uint64_t>(v, s_wave, grand);
+  }
+}
+
+// One onesweep launch is either one digit pass of a plain sort (SEG == false: tile t =
+// blockIdx.x holds items [t*TILE, (t+1)*TILE) and its digit starts come from the spine)
+// or one digit pass over every large segment of a batched segmented sort (SEG == true:
+// tiles are grouped per segment, the tile range of segment slot s starts at
+// slot_first_tile[s], and a decoupled look-back that stops at the segment's first tile
+// gives the digit starts).
 struct OnesweepParams
 {
-  const void* keys_in; // plain: first item of this portion; SEG: whole array
-  void* keys_out; // whole output array
+  const void* keys_in;
+  void* keys_out;
   const void* vals_in;
   void* vals_out;
-  uint32_t* lookback; // [tiles][kRadix], zero on entry
-  uint32_t* lookback_next; // cleared here for the next launch (may be null)
-  uint32_t* tile_counter; // zero on entry
-  const uint64_t* bins; // plain: first output position of every digit for this portion
-  uint64_t* bins_next; // plain: last tile writes digit starts of the next portion (may be null)
-  uint64_t num_items; // plain: items in this portion (< 2^30)
+  uint64_t num_items; // plain: items of the sort
   uint64_t xor_mask;
   uint32_t num_tiles; // tiles of this launch (SEG: the grid size, an upper bound)
-  uint32_t zero_tiles; // tiles of lookback_next to clear
   int float_mode;
   int shift;
   int num_bits;
   int small_offsets; // output arrays below 4 GiB: 32-bit byte offsets
 
+  // plain only
+  const uint32_t* tile_off; // [num_tiles][kRadix]
+  const uint64_t* block_prefix; // [kRadix][num_blocks]
+  const uint64_t* digit_start; // [kRadix]
+  uint32_t tiles_per_block;
+  uint32_t num_blocks;
+  uint32_t num_xcds; // workgroup w runs on XCD w % num_xcds
+
   // SEG only
+  uint32_t* lookback; // [tiles][kRadix], zero on entry
+  uint32_t* lookback_next; // cleared here for the next launch (may be null)
+  uint32_t* tile_counter; // zero on entry
+  uint32_t zero_tiles; // tiles of lookback_next to clear
   const uint32_t* tile_slot; // segment slot of each tile
   const uint32_t* slot_first_tile; // first tile of each slot
   const int64_t* slot_range; // [slot][2] = begin, end item offsets
@@ -293,35 +458,54 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
   uint32_t* wc         = s_wcnt + wave * kRadix;
   const TwiddleT<UK, FLOAT> tw{UK(p.xor_mask)};
   const uint32_t dmask = (1u << p.num_bits) - 1u;
-  const uint32_t limit = SEG ? *p.total_tiles : p.num_tiles;
   UK* kout             = static_cast<UK*>(p.keys_out);
   VS* vout             = static_cast<VS*>(p.vals_out);
 
-  // Tile ids follow the order in which workgroups start (an atomic counter, not
-  // blockIdx), so every tile a look-back waits on belongs to a workgroup that is already
-  // running.
-  if (tid == 0)
-  {
-    s_misc[1] = __hip_atomic_fetch_add(p.tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
   zero_wave_counters(wc);
-  __syncthreads();
-  const uint32_t tile = s_misc[1];
-  if (tile >= limit)
+  uint32_t tile  = blockIdx.x;
+  uint64_t gbase = 0; // plain, thread d < kRadix: first output position of digit d in this tile
+  if constexpr (SEG)
   {
-    return; // block-uniform; only a segmented launch has spare workgroups
-  }
-
-  // Clear this tile's entries of the other look-back buffer for the next launch; that
-  // buffer is not read by this launch.
-  if (p.lookback_next != nullptr)
-  {
-    for (uint64_t t = tile; t < p.zero_tiles; t += p.num_tiles)
+    // Tile ids follow the order in which workgroups start (an atomic counter, not
+    // blockIdx), so every tile a look-back waits on belongs to a workgroup that is
+    // already running.
+    if (tid == 0)
     {
-      for (uint32_t i = tid; i < uint32_t(kRadix); i += THREADS)
+      s_misc[1] = __hip_atomic_fetch_add(p.tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    lds_barrier();
+    tile = s_misc[1];
+    if (tile >= *p.total_tiles)
+    {
+      return; // block-uniform: spare workgroup
+    }
+    // Clear this tile's entries of the other look-back buffer for the next launch;
+    // that buffer is not read by this launch.
+    if (p.lookback_next != nullptr)
+    {
+      for (uint64_t t = tile; t < p.zero_tiles; t += p.num_tiles)
       {
-        p.lookback_next[t * kRadix + i] = 0u;
+        for (uint32_t i = tid; i < uint32_t(kRadix); i += THREADS)
+        {
+          p.lookback_next[t * kRadix + i] = 0u;
+        }
       }
+    }
+  }
+  else
+  {
+    // Workgroups are dispatched to the XCDs round-robin; XCD x takes the x-th contiguous
+    // range of tiles, in order.  Consecutive tiles then run on one XCD at about the same
+    // time, and their adjacent digit runs meet in that XCD's L2 as whole lines instead
+    // of as partial lines from different L2s.
+    const uint32_t nx = p.num_xcds;
+    const uint32_t x  = blockIdx.x % nx;
+    tile              = x * (p.num_tiles / nx) + min(x, p.num_tiles % nx) + blockIdx.x / nx;
+    // Issued first: these loads complete while the keys are ranked.
+    if (tid < uint32_t(kRadix))
+    {
+      gbase = p.digit_start[tid] + p.block_prefix[uint64_t(tid) * p.num_blocks + tile / p.tiles_per_block]
+            + p.tile_off[uint64_t(tile) * kRadix + tid];
     }
   }
 
@@ -403,7 +587,7 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
       }
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // Thread d (< kRadix) owns digit d: the tile count, published at once for the
   // look-back of later tiles, then the exclusive scan over digits, then each wave's
@@ -421,22 +605,22 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
   }
   const uint32_t pad         = uint32_t(kTile) - valid;
   const uint32_t valid_total = (tid == uint32_t(kRadix - 1)) ? total - pad : total;
-  if (tid < uint32_t(kRadix))
+  if constexpr (SEG)
   {
-    store_status(&p.lookback[uint64_t(tile) * kRadix + tid], (tile == first_tile ? kFlagGlobal : kFlagPartial) | valid_total);
+    if (tid < uint32_t(kRadix))
+    {
+      store_status(&p.lookback[uint64_t(tile) * kRadix + tid],
+                   (tile == first_tile ? kFlagGlobal : kFlagPartial) | valid_total);
+    }
   }
   uint32_t tile_items  = 0;
-  const uint32_t start = radix_exclusive_scan<uint32_t>(total, s_misc + 4, tile_items);
-  uint64_t gstart      = 0; // thread d < kRadix: first output position of digit d
+  const uint32_t start = radix_exclusive_scan<uint32_t, true>(total, s_misc + 4, tile_items);
+  uint64_t gstart      = 0; // SEG, thread d < kRadix: first output position of digit d in the segment
   if constexpr (SEG)
   {
     const uint32_t* sb = p.seg_bins + (uint64_t(slot) * p.num_passes + p.pass) * kRadix;
     uint64_t all       = 0;
-    gstart = seg_begin + radix_exclusive_scan<uint64_t>(tid < uint32_t(kRadix) ? uint64_t(sb[tid]) : 0, s_scan64, all);
-  }
-  else
-  {
-    gstart = tid < uint32_t(kRadix) ? p.bins[tid] : 0;
+    gstart = seg_begin + radix_exclusive_scan<uint64_t, true>(tid < uint32_t(kRadix) ? uint64_t(sb[tid]) : 0, s_scan64, all);
   }
   if (tid < uint32_t(kRadix))
   {
@@ -447,8 +631,12 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
       s_wcnt[w * kRadix + tid] = run;
       run += c[w];
     }
+    if constexpr (!SEG)
+    {
+      s_base[tid] = gbase - start;
+    }
   }
-  __syncthreads();
+  lds_barrier();
 
   // Stage keys and values in LDS in digit order.
 #pragma unroll
@@ -462,25 +650,21 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
     }
   }
 
-  // Decoupled look-back: walk earlier tiles, adding their counts, until one carries
-  // the inclusive prefix.
-  if (tid < uint32_t(kRadix))
+  if constexpr (SEG)
   {
-    const uint32_t excl = lookback_window(p.lookback, tile, first_tile, tid);
-    if (tile > first_tile)
+    // Decoupled look-back: walk earlier tiles of the segment, adding their counts,
+    // until one carries the inclusive prefix.
+    if (tid < uint32_t(kRadix))
     {
-      store_status(&p.lookback[uint64_t(tile) * kRadix + tid], kFlagGlobal | (excl + valid_total));
-    }
-    s_base[tid] = gstart + excl - start;
-    if constexpr (!SEG)
-    {
-      if (p.bins_next != nullptr && tile == p.num_tiles - 1u)
+      const uint32_t excl = lookback_window(p.lookback, tile, first_tile, tid);
+      if (tile > first_tile)
       {
-        p.bins_next[tid] = gstart + excl + valid_total;
+        store_status(&p.lookback[uint64_t(tile) * kRadix + tid], kFlagGlobal | (excl + valid_total));
       }
+      s_base[tid] = gstart + excl - start;
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // Write out in LDS order: consecutive threads take consecutive staged items, which
   // land at consecutive addresses within each digit's run.

@@ -209,11 +209,11 @@ def default_config(key_bytes: int, value_bytes: int, num_items: int) -> B.radix_
 
 
 def make_config(onesweep_threads, onesweep_ipt, single_tile_threads=256, single_tile_ipt=16,
-                histogram_blocks_per_cu=4) -> B.radix_sort_config:
+                upsweep_blocks_per_cu=8) -> B.radix_sort_config:
     c = B.radix_sort_config()
     c.onesweep_threads, c.onesweep_ipt, c.radix_bits = onesweep_threads, onesweep_ipt, 8
     c.single_tile_threads, c.single_tile_ipt = single_tile_threads, single_tile_ipt
-    c.histogram_blocks_per_cu = histogram_blocks_per_cu
+    c.upsweep_blocks_per_cu = upsweep_blocks_per_cu
     return c
 
 

@@ -20,8 +20,10 @@ CCCL_C_EXTERN_C_BEGIN
 
 /* Launch configuration of the device-wide sort.  Plays the role of CUB's
  * RadixSortPolicy (cub/cub/device/dispatch/tuning/tuning_radix_sort.cuh:255-293):
- * onesweep tile = onesweep_threads * onesweep_ipt items; inputs of at most
- * single_tile_threads * single_tile_ipt items are sorted by one workgroup. */
+ * a digit pass works on tiles of onesweep_threads * onesweep_ipt items; the upsweep
+ * gives each of about upsweep_blocks_per_cu * (compute units) workgroups a block of
+ * consecutive tiles; inputs of at most single_tile_threads * single_tile_ipt items are
+ * sorted by one workgroup. */
 typedef struct cccl_amd_radix_sort_config_t
 {
   int32_t onesweep_threads;
@@ -29,7 +31,7 @@ typedef struct cccl_amd_radix_sort_config_t
   int32_t radix_bits; /* digit width; 8 is the only compiled width */
   int32_t single_tile_threads; /* 0 disables the one-workgroup path */
   int32_t single_tile_ipt;
-  int32_t histogram_blocks_per_cu;
+  int32_t upsweep_blocks_per_cu;
   int32_t reserved[2];
 } cccl_amd_radix_sort_config_t;
 

@@ -58,9 +58,10 @@ def test_build_load_cleanup():
     rc, br = _build()
     assert rc == B.CUDA_SUCCESS
     assert br.payload and br.payload_size > 0 and br.runtime_policy and br.runtime_policy_size > 0
-    assert br.onesweep_kernel and br.histogram_kernel and br.single_tile_kernel and br.init_bins_and_counters_kernel
+    assert br.downsweep_kernel and br.upsweep_kernel and br.scan_bins_kernel and br.single_tile_kernel
+    assert br.init_bins_and_counters_kernel and not br.onesweep_kernel
     assert br.key_type.size == 4 and br.value_type.size == 4 and br.order == B.CCCL_ASCENDING
-    assert b"onesweep" in br.onesweep_kernel_lowered_name
+    assert b"upsweep" in br.upsweep_kernel_lowered_name and b"spine" in br.scan_bins_kernel_lowered_name
     assert L.cccl_device_radix_sort_cleanup(ctypes.byref(br)) == B.CUDA_SUCCESS
     assert not br.payload and not br.runtime_policy
 
@@ -106,12 +107,12 @@ def test_serialize_round_trip_then_load():
     br2 = B.cccl_device_radix_sort_build_result_t()
     assert L.cccl_device_radix_sort_deserialize(ctypes.byref(br2), buf, size) == 0
     L.cccl_serialization_buffer_free(buf)
-    assert not br2.onesweep_kernel  # handles come from a later load
+    assert not br2.downsweep_kernel  # handles come from a later load
     assert br2.order == B.CCCL_DESCENDING and br2.key_type.type == B.CCCL_FLOAT32 and br2.value_type.size == 8
-    assert br2.onesweep_kernel_lowered_name == br.onesweep_kernel_lowered_name
+    assert br2.downsweep_kernel_lowered_name == br.downsweep_kernel_lowered_name
     assert ctypes.string_at(br2.payload, br2.payload_size) == ctypes.string_at(br.payload, br.payload_size)
     assert L.cccl_device_radix_sort_load(ctypes.byref(br2)) == 0
-    assert br2.onesweep_kernel
+    assert br2.downsweep_kernel
     # a corrupted blob is rejected: bad order field
     bad = bytearray(blob)
     off = 8 + 4 + 4 + 20 + 20
