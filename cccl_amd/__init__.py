@@ -5,6 +5,8 @@ cccl_device_radix_sort_* C-ABI; this package is its Python host binding.
 """
 
 from .radix_sort import (  # noqa: F401
+    ALGO_ONESWEEP,
+    ALGO_PASSES,
     DoubleBuffer,
     SegmentedSorter,
     SortError,
@@ -17,6 +19,7 @@ from .radix_sort import (  # noqa: F401
     device_info,
     make_config,
     make_radix_sort,
+    onesweep_lookback_shapes,
     onesweep_shapes,
     radix_sort,
     segmented_radix_sort,

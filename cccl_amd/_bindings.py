@@ -107,7 +107,8 @@ class radix_sort_config(ctypes.Structure):
         ("single_tile_threads", ctypes.c_int32),
         ("single_tile_ipt", ctypes.c_int32),
         ("upsweep_blocks_per_cu", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("algorithm", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
     def as_tuple(self):
@@ -146,6 +147,7 @@ EXPORTED_SYMBOLS = [
     "cccl_amd_segmented_default_config",
     "cccl_amd_radix_sort_config_valid",
     "cccl_amd_onesweep_shapes",
+    "cccl_amd_onesweep_lookback_shapes",
     "cccl_amd_block_sort_shapes",
     "cccl_amd_profile_enable",
     "cccl_amd_profile_read",
@@ -211,6 +213,8 @@ def lib():
     L.cccl_amd_radix_sort_config_valid.restype = i32
     L.cccl_amd_onesweep_shapes.argtypes = [ctypes.POINTER(i32), ctypes.POINTER(i32), i32]
     L.cccl_amd_onesweep_shapes.restype = i32
+    L.cccl_amd_onesweep_lookback_shapes.argtypes = [ctypes.POINTER(i32), ctypes.POINTER(i32), i32]
+    L.cccl_amd_onesweep_lookback_shapes.restype = i32
     L.cccl_amd_block_sort_shapes.argtypes = [ctypes.POINTER(i32), ctypes.POINTER(i32), i32]
     L.cccl_amd_block_sort_shapes.restype = i32
     L.cccl_amd_profile_enable.argtypes = [i32]

@@ -800,6 +800,20 @@ CCCL_C_API int cccl_amd_radix_sort_config_valid(const cccl_amd_radix_sort_config
   return cfg != nullptr && cccl_amd::config_is_valid(*cfg, key_bytes, value_bytes) ? 1 : 0;
 }
 
+namespace {
+
+int copy_shapes(const int* t, const int* i, int n, int* threads, int* ipt, int max)
+{
+  for (int k = 0; k < n && k < max; ++k)
+  {
+    threads[k] = t[k];
+    ipt[k]     = i[k];
+  }
+  return n;
+}
+
+} // namespace
+
 CCCL_C_API int cccl_amd_onesweep_shapes(int* threads, int* ipt, int max)
 {
   static const int t[] = {
@@ -812,13 +826,22 @@ CCCL_C_API int cccl_amd_onesweep_shapes(int* threads, int* ipt, int max)
     CCCL_AMD_ONESWEEP_SHAPES(CCCL_AMD_X)
 #undef CCCL_AMD_X
   };
-  const int n = int(sizeof(t) / sizeof(t[0]));
-  for (int k = 0; k < n && k < max; ++k)
-  {
-    threads[k] = t[k];
-    ipt[k]     = i[k];
-  }
-  return n;
+  return copy_shapes(t, i, int(sizeof(t) / sizeof(t[0])), threads, ipt, max);
+}
+
+CCCL_C_API int cccl_amd_onesweep_lookback_shapes(int* threads, int* ipt, int max)
+{
+  static const int t[] = {
+#define CCCL_AMD_X(T, I) T,
+    CCCL_AMD_SEG_ONESWEEP_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
+  static const int i[] = {
+#define CCCL_AMD_X(T, I) I,
+    CCCL_AMD_SEG_ONESWEEP_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
+  return copy_shapes(t, i, int(sizeof(t) / sizeof(t[0])), threads, ipt, max);
 }
 
 CCCL_C_API const char* cccl_amd_last_error_message(void)
@@ -869,13 +892,7 @@ CCCL_C_API int cccl_amd_block_sort_shapes(int* threads, int* ipt, int max)
     CCCL_AMD_BLOCK_SORT_SHAPES(CCCL_AMD_X)
 #undef CCCL_AMD_X
   };
-  const int n = int(sizeof(t) / sizeof(t[0]));
-  for (int k = 0; k < n && k < max; ++k)
-  {
-    threads[k] = t[k];
-    ipt[k]     = i[k];
-  }
-  return n;
+  return copy_shapes(t, i, int(sizeof(t) / sizeof(t[0])), threads, ipt, max);
 }
 
 } // extern "C"

@@ -334,8 +334,24 @@ bool config_is_valid(const cccl_amd_radix_sort_config_t& cfg, int key_bytes, int
   {
     return false;
   }
-  if (find_entry(tab->onesweep, tab->num_onesweep, key_bytes, val_bytes, cfg.onesweep_threads, cfg.onesweep_ipt)
-      == nullptr)
+  if (cfg.algorithm == CCCL_AMD_ALGO_PASSES)
+  {
+    if (find_entry(tab->onesweep, tab->num_onesweep, key_bytes, val_bytes, cfg.onesweep_threads, cfg.onesweep_ipt)
+        == nullptr)
+    {
+      return false;
+    }
+  }
+  else if (cfg.algorithm == CCCL_AMD_ALGO_ONESWEEP)
+  {
+    if (find_entry(
+          tab->seg_onesweep, tab->num_seg_onesweep, key_bytes, val_bytes, cfg.onesweep_threads, cfg.onesweep_ipt)
+        == nullptr)
+    {
+      return false;
+    }
+  }
+  else
   {
     return false;
   }
@@ -348,6 +364,66 @@ bool config_is_valid(const cccl_amd_radix_sort_config_t& cfg, int key_bytes, int
   }
   return true;
 }
+
+namespace {
+
+// CUB's onesweep (invoke_onesweep, dispatch_radix_sort.cuh:1698-1948): one upfront
+// histogram of every digit pass, then one kernel per pass whose tiles take their ids
+// from an atomic counter and find their digit offsets by decoupled look-back.  The
+// batched segmented sort already runs exactly that over each large segment, so the
+// whole input goes through it as one segment [0, n).
+hipError_t onesweep_lookback_sort(
+  void* d_temp_storage,
+  size_t& temp_storage_bytes,
+  DoubleBufferRef& keys,
+  DoubleBufferRef& vals,
+  const SortProblem& pr,
+  const cccl_amd_radix_sort_config_t& cfg,
+  hipStream_t stream)
+{
+  if (pr.num_items > kMaxPortionItems)
+  {
+    set_last_error("the onesweep algorithm sorts at most 2^30 - 1 items (30-bit look-back counts); got %llu",
+                   static_cast<unsigned long long>(pr.num_items));
+    return hipErrorInvalidValue;
+  }
+  cccl_amd_segmented_config_t sc{};
+  sc.num_block_classes = 0;
+  sc.large_threads     = cfg.onesweep_threads;
+  sc.large_ipt         = cfg.onesweep_ipt;
+  sc.strategy          = 0;
+  size_t seg_bytes     = 0;
+  {
+    DoubleBufferRef k = keys;
+    DoubleBufferRef v = vals;
+    if (hipError_t e = segmented_radix_sort(nullptr, seg_bytes, k, v, pr, 1, nullptr, nullptr, 8, sc, stream);
+        e != hipSuccess)
+    {
+      return e;
+    }
+  }
+  size_t sizes[2] = {2 * sizeof(int64_t), seg_bytes};
+  void* parts[2]  = {};
+  if (hipError_t e = alias_temporaries(d_temp_storage, temp_storage_bytes, parts, sizes, 2); e != hipSuccess)
+  {
+    return e;
+  }
+  if (d_temp_storage == nullptr)
+  {
+    return hipSuccess;
+  }
+  int64_t* d_offsets = static_cast<int64_t*>(parts[0]);
+  SingleSegmentParams sp{d_offsets, int64_t(pr.num_items)};
+  if (hipError_t e =
+        launch_kernel(reinterpret_cast<const void*>(&single_segment_kernel), 1, 64, 0, stream, &sp, 0);
+      e != hipSuccess)
+  {
+    return e;
+  }
+  return segmented_radix_sort(parts[1], seg_bytes, keys, vals, pr, 1, d_offsets, d_offsets + 1, 8, sc, stream);
+}
+
+} // namespace
 
 hipError_t radix_sort(
   void* d_temp_storage,
@@ -448,6 +524,16 @@ hipError_t radix_sort(
     return hipSuccess;
   }
 
+  if (cfg.algorithm == CCCL_AMD_ALGO_ONESWEEP)
+  {
+    return onesweep_lookback_sort(d_temp_storage, temp_storage_bytes, keys, vals, pr, cfg, stream);
+  }
+  if (cfg.algorithm != CCCL_AMD_ALGO_PASSES)
+  {
+    set_last_error("unknown algorithm %d", cfg.algorithm);
+    return hipErrorInvalidValue;
+  }
+
   // Digit passes (InvokePasses, dispatch_radix_sort.cuh:832-990): each pass is an
   // upsweep (per-tile digit counts), a spine (scan of the counts) and a downsweep (rank
   // and scatter), so no tile of a pass waits on another.
@@ -468,10 +554,10 @@ hipError_t radix_sort(
     return hipErrorInvalidValue;
   }
   // Blocks of consecutive tiles for the upsweep: about upsweep_blocks_per_cu per compute
-  // unit, each holding fewer than 2^32 items (block sums are 32-bit).
+  // unit, at most kMaxTilesPerBlock tiles each (block sums stay below 2^32).
   const uint64_t want_blocks = uint64_t(device_cu_count()) * uint64_t(std::max(1, cfg.upsweep_blocks_per_cu));
-  const uint64_t max_tpb     = (0xffffffffull / tile);
-  const uint64_t tpb         = std::max<uint64_t>(1, std::min(ceil_div(tiles, want_blocks), max_tpb));
+  const uint64_t tpb =
+    std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(tiles, want_blocks), kMaxTilesPerBlock));
   const uint64_t blocks      = ceil_div(tiles, tpb);
   const bool need_tmp        = !pr.overwrite && passes > 1;
 
@@ -536,7 +622,9 @@ hipError_t radix_sort(
     upp.float_mode      = float_mode;
     upp.shift           = bit;
     upp.num_bits        = num_bits;
-    if (hipError_t e = launch_kernel(up->fn, blocks, up->threads, 0, stream, &upp, 1); e != hipSuccess)
+    if (hipError_t e = launch_kernel(up->fn, blocks, kUpsweepThreads, size_t(tpb) * kRadix * sizeof(uint32_t), stream,
+                                     &upp, 1);
+        e != hipSuccess)
     {
       return e;
     }

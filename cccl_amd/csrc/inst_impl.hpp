@@ -40,7 +40,7 @@ KernelEntry onesweep_entry()
 template <class UK, int T, int I>
 KernelEntry upsweep_entry()
 {
-  return {int(sizeof(UK)), 0, T, I, reinterpret_cast<const void*>(&upsweep_kernel<UK, T, I>), 0};
+  return {int(sizeof(UK)), 0, T, I, reinterpret_cast<const void*>(&upsweep_kernel<UK, T * I>), 0};
 }
 
 template <class UK, class V, int T, int I>

@@ -57,11 +57,13 @@ const KeyWidthTable* kernels_for_key_bytes(int key_bytes);
   X(1024, 8)                        \
   X(1024, 16)
 
-// Launch shapes compiled for the batched onesweep of large segments.
+// Launch shapes compiled for the onesweep with decoupled look-back: the batched onesweep
+// of large segments, the per-segment passes, and the plain sort's onesweep algorithm.
 #define CCCL_AMD_SEG_ONESWEEP_SHAPES(X) \
   X(256, 16)                            \
   X(512, 8)                             \
-  X(512, 16)
+  X(512, 16)                            \
+  X(1024, 16)
 
 // Launch shapes compiled for the single-workgroup sort.
 #define CCCL_AMD_BLOCK_SORT_SHAPES(X) \

@@ -39,6 +39,23 @@ static __global__ void __launch_bounds__(256) init_kernel(InitParams p)
   }
 }
 
+// The one segment [0, n) through which the plain sort's onesweep algorithm runs the
+// batched segmented pass: offsets[0] = begin, offsets[1] = end.
+struct SingleSegmentParams
+{
+  int64_t* offsets;
+  int64_t num_items;
+};
+
+static __global__ void __launch_bounds__(64) single_segment_kernel(SingleSegmentParams p)
+{
+  if (threadIdx.x == 0)
+  {
+    p.offsets[0] = 0;
+    p.offsets[1] = p.num_items;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Plain sort, one digit pass = upsweep + spine + downsweep
 // ---------------------------------------------------------------------------
@@ -74,189 +91,161 @@ struct UpsweepParams
   int num_bits;
 };
 
-// One workgroup per block of tiles.  Each wave counts into its own LDS counters; the
-// running sums at each tile boundary are that tile's tile_off row.  A wave whose 64
-// digits are all equal (low-entropy keys) adds them with one atomic instead of 64
-// conflicting ones.
-template <class UK, int THREADS, int IPT, bool FLOAT>
+// Largest number of tiles one upsweep workgroup counts (its LDS holds one row of
+// counters per tile).
+constexpr uint32_t kMaxTilesPerBlock = 16;
+constexpr int kUpsweepThreads        = 256;
+
+// One workgroup per block of at most kMaxTilesPerBlock consecutive tiles.  The block's
+// keys are streamed as one range (tile boundaries need no barrier): every key adds one
+// to the LDS counter of (its tile, its digit).  A 16-byte aligned array is read with
+// 16-byte loads, four per thread per batch, the next batch in flight while the current
+// one is counted; a wave's 64 loads are 1 KiB of one tile.  When all 64 lanes hit the
+// same counter (low-entropy keys) one lane adds 64 instead of 64 conflicting atomics.
+// At the end each digit's counts are scanned over the block's tiles into tile_off rows
+// and the block sum.
+template <class UK, int TILE, bool FLOAT>
 __device__ __forceinline__ void upsweep_block(const UpsweepParams& p, uint32_t* s_cnt)
 {
-  constexpr int kWaves = THREADS / 64;
-  constexpr int kTile  = THREADS * IPT;
-  const uint32_t tid   = threadIdx.x;
-  uint32_t* wc         = s_cnt + (tid >> 6) * kRadix;
+  constexpr int kThreads = kUpsweepThreads;
+  const uint32_t tid     = threadIdx.x;
   const TwiddleT<UK, FLOAT> tw{UK(p.xor_mask)};
-  const uint32_t dmask = (1u << p.num_bits) - 1u;
-  const UK* keys       = static_cast<const UK*>(p.keys);
-  const uint32_t b     = blockIdx.x;
-  const uint32_t t0    = b * p.tiles_per_block;
-  const uint32_t t1    = min(t0 + p.tiles_per_block, p.num_tiles);
+  const uint32_t dmask   = (1u << p.num_bits) - 1u;
+  const UK* keys         = static_cast<const UK*>(p.keys);
+  const uint32_t b       = blockIdx.x;
+  const uint32_t t0      = b * p.tiles_per_block;
+  const uint32_t t1      = min(t0 + p.tiles_per_block, p.num_tiles);
+  const uint32_t ntiles  = t1 - t0;
+  const uint64_t begin   = uint64_t(t0) * TILE;
+  const uint64_t end     = min<uint64_t>(uint64_t(t1) * TILE, p.num_items);
 
-  zero_wave_counters(wc);
-  auto count = [&](UK k) {
-    const uint32_t d  = extract_digit(tw(k), p.shift, dmask);
-    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-    const uint64_t eq = __ballot(d == d0);
+  for (uint32_t i = tid; i < ntiles * kRadix; i += kThreads)
+  {
+    s_cnt[i] = 0;
+  }
+  lds_barrier();
+
+  // idx = tile within the block * kRadix + digit
+  auto add = [&](UK k, uint32_t tile_local) {
+    const uint32_t idx  = tile_local * kRadix + extract_digit(tw(k), p.shift, dmask);
+    const uint32_t idx0 = __builtin_amdgcn_readfirstlane(idx);
+    const uint64_t eq   = __ballot(idx == idx0);
     if (eq == __builtin_amdgcn_read_exec()) // wave-uniform
     {
       if (mask_prefix_count(eq) == 0u)
       {
-        __hip_atomic_fetch_add(wc + d0, uint32_t(__builtin_popcountll(eq)), __ATOMIC_RELAXED,
+        __hip_atomic_fetch_add(s_cnt + idx0, uint32_t(__builtin_popcountll(eq)), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     }
     else
     {
-      __hip_atomic_fetch_add(wc + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(s_cnt + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   };
 
-  // Row t of tile_off: the running counts before tile t.
-  auto tile_start = [&](uint32_t t) {
-    lds_barrier(); // the counts of the block's earlier tiles are complete
-    if (tid < uint32_t(kRadix))
-    {
-      uint32_t s = 0;
-#pragma unroll
-      for (int w = 0; w < kWaves; ++w)
-      {
-        s += s_cnt[w * kRadix + tid];
-      }
-      p.tile_off[uint64_t(t) * kRadix + tid] = s;
-    }
-    lds_barrier(); // read before any of this tile's counts are added
-  };
-
-  // Full tiles: tile t + 1 is loaded while tile t is counted, from two register sets
-  // used alternately (no copies between them, so the loads stay in flight).  Counting
-  // does not care which thread holds which key, so a 16-byte aligned array is read with
-  // 16-byte loads.
-  const uint32_t full_end = (t1 == p.num_tiles && p.num_items % kTile != 0) ? t1 - 1 : t1;
-  auto stream_full = [&](auto loader) {
-    UK ka[IPT];
-    UK kb[IPT];
-    uint32_t t = t0;
-    if (t < full_end)
-    {
-      loader(t, ka);
-    }
-    while (t < full_end)
-    {
-      if (t + 1 < full_end)
-      {
-        loader(t + 1, kb);
-      }
-      tile_start(t);
-#pragma unroll
-      for (int i = 0; i < IPT; ++i)
-      {
-        count(ka[i]);
-      }
-      if (++t >= full_end)
-      {
-        break;
-      }
-      if (t + 1 < full_end)
-      {
-        loader(t + 1, ka);
-      }
-      tile_start(t);
-#pragma unroll
-      for (int i = 0; i < IPT; ++i)
-      {
-        count(kb[i]);
-      }
-      ++t;
-    }
-  };
-  bool vector_path = false;
-  if constexpr ((IPT * sizeof(UK)) % 16 == 0)
+  uint64_t scalar_from = begin; // keys before this position have been counted
+  if constexpr (TILE * sizeof(UK) % 1024 == 0)
   {
-    constexpr int kVec = int(IPT * sizeof(UK) / 16);
-    constexpr int kPer = int(16 / sizeof(UK));
     if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0u)
     {
-      vector_path = true;
-      stream_full([&](uint32_t t, UK(&k)[IPT]) {
-        const uint4* src = reinterpret_cast<const uint4*>(keys + uint64_t(t) * kTile);
+      constexpr int kPer               = int(16 / sizeof(UK));
+      constexpr uint32_t kVecPerTile   = uint32_t(TILE / kPer);
+      constexpr int kU                 = 4;
+      constexpr uint32_t kStep         = uint32_t(kU * kThreads);
+      const uint4* src                 = reinterpret_cast<const uint4*>(keys + begin);
+      const uint32_t nvec              = uint32_t((end - begin) / kPer);
+      auto load_batch = [&](uint32_t base, uint4(&x)[kU]) {
 #pragma unroll
-        for (int j = 0; j < kVec; ++j)
+        for (int u = 0; u < kU; ++u)
         {
-          const uint4 v = src[j * THREADS + tid];
-          UK e[kPer];
-          __builtin_memcpy(e, &v, 16);
+          const uint32_t v = base + uint32_t(u * kThreads) + tid;
+          x[u]             = v < nvec ? src[v] : uint4{0u, 0u, 0u, 0u};
+        }
+      };
+      auto count_batch = [&](uint32_t base, const uint4(&x)[kU]) {
 #pragma unroll
-          for (int q = 0; q < kPer; ++q)
+        for (int u = 0; u < kU; ++u)
+        {
+          const uint32_t v = base + uint32_t(u * kThreads) + tid;
+          if (v < nvec)
           {
-            k[j * kPer + q] = e[q];
+            UK e[kPer];
+            __builtin_memcpy(e, &x[u], 16);
+            const uint32_t tl = v / kVecPerTile; // wave-uniform
+#pragma unroll
+            for (int q = 0; q < kPer; ++q)
+            {
+              add(e[q], tl);
+            }
           }
         }
-      });
+      };
+      uint4 xa[kU];
+      uint4 xb[kU];
+      if (nvec > 0u)
+      {
+        load_batch(0u, xa);
+      }
+      for (uint32_t base = 0; base < nvec; base += 2u * kStep)
+      {
+        if (base + kStep < nvec)
+        {
+          load_batch(base + kStep, xb);
+        }
+        count_batch(base, xa);
+        if (base + kStep >= nvec)
+        {
+          break;
+        }
+        if (base + 2u * kStep < nvec)
+        {
+          load_batch(base + 2u * kStep, xa);
+        }
+        count_batch(base + kStep, xb);
+      }
+      scalar_from = begin + uint64_t(nvec) * kPer;
     }
   }
-  if (!vector_path)
+  // Unaligned arrays, and the last few keys of an array whose length is not a multiple
+  // of the vector width: one key per thread per step.
+  for (uint64_t i = scalar_from + tid; i < end; i += kThreads)
   {
-    stream_full([&](uint32_t t, UK(&k)[IPT]) {
-      const UK* src = keys + uint64_t(t) * kTile;
-#pragma unroll
-      for (int i = 0; i < IPT; ++i)
-      {
-        k[i] = src[i * THREADS + tid];
-      }
-    });
-  }
-  // The partial last tile of the array, item by item (item i*THREADS + tid).
-  if (full_end < t1)
-  {
-    const uint64_t base  = uint64_t(full_end) * kTile;
-    const uint32_t valid = uint32_t(p.num_items - base);
-    UK k[IPT];
-#pragma unroll
-    for (int i = 0; i < IPT; ++i)
-    {
-      const uint32_t pos = uint32_t(i) * THREADS + tid;
-      k[i]               = pos < valid ? keys[base + pos] : UK(0);
-    }
-    tile_start(full_end);
-#pragma unroll
-    for (int i = 0; i < IPT; ++i)
-    {
-      if (uint32_t(i) * THREADS + tid < valid)
-      {
-        count(k[i]);
-      }
-    }
+    add(keys[i], uint32_t((i - begin) / TILE));
   }
   lds_barrier();
+
   if (tid < uint32_t(kRadix))
   {
-    uint32_t s = 0;
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w)
+    uint32_t run = 0;
+    for (uint32_t tl = 0; tl < ntiles; ++tl)
     {
-      s += s_cnt[w * kRadix + tid];
+      const uint32_t c                                   = s_cnt[tl * kRadix + tid];
+      p.tile_off[uint64_t(t0 + tl) * kRadix + tid] = run;
+      run += c;
     }
-    p.block_sum[uint64_t(tid) * p.num_blocks + b] = s;
+    p.block_sum[uint64_t(tid) * p.num_blocks + b] = run;
   }
 }
 
-template <class UK, int THREADS, int IPT>
-__global__ void __launch_bounds__(THREADS) upsweep_kernel(UpsweepParams p)
+template <class UK, int TILE>
+__global__ void __launch_bounds__(kUpsweepThreads) upsweep_kernel(UpsweepParams p)
 {
-  __shared__ uint32_t s_cnt[(THREADS / 64) * kRadix];
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(smem); // [tiles_per_block][kRadix]
   if constexpr (sizeof(UK) == 1)
   {
-    upsweep_block<UK, THREADS, IPT, false>(p, s_cnt);
+    upsweep_block<UK, TILE, false>(p, s_cnt);
   }
   else
   {
     if (p.float_mode)
     {
-      upsweep_block<UK, THREADS, IPT, true>(p, s_cnt);
+      upsweep_block<UK, TILE, true>(p, s_cnt);
     }
     else
     {
-      upsweep_block<UK, THREADS, IPT, false>(p, s_cnt);
+      upsweep_block<UK, TILE, false>(p, s_cnt);
     }
   }
 }

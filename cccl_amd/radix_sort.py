@@ -208,12 +208,17 @@ def default_config(key_bytes: int, value_bytes: int, num_items: int) -> B.radix_
     return c
 
 
+ALGO_PASSES = 0  # upsweep + bin scan + downsweep per digit pass
+ALGO_ONESWEEP = 1  # upfront histogram + one decoupled-look-back kernel per digit pass
+
+
 def make_config(onesweep_threads, onesweep_ipt, single_tile_threads=256, single_tile_ipt=16,
-                upsweep_blocks_per_cu=8) -> B.radix_sort_config:
+                upsweep_blocks_per_cu=8, algorithm=ALGO_PASSES) -> B.radix_sort_config:
     c = B.radix_sort_config()
     c.onesweep_threads, c.onesweep_ipt, c.radix_bits = onesweep_threads, onesweep_ipt, 8
     c.single_tile_threads, c.single_tile_ipt = single_tile_threads, single_tile_ipt
     c.upsweep_blocks_per_cu = upsweep_blocks_per_cu
+    c.algorithm = algorithm
     return c
 
 
@@ -225,6 +230,15 @@ def onesweep_shapes() -> list[tuple[int, int]]:
     t = (ctypes.c_int * 64)()
     i = (ctypes.c_int * 64)()
     n = B.lib().cccl_amd_onesweep_shapes(t, i, 64)
+    return [(t[k], i[k]) for k in range(n)]
+
+
+def onesweep_lookback_shapes() -> list[tuple[int, int]]:
+    """Tile shapes of the onesweep algorithm (ALGO_ONESWEEP) and of the segmented
+    sort's large segments."""
+    t = (ctypes.c_int * 64)()
+    i = (ctypes.c_int * 64)()
+    n = B.lib().cccl_amd_onesweep_lookback_shapes(t, i, 64)
     return [(t[k], i[k]) for k in range(n)]
 
 

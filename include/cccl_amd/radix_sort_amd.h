@@ -23,7 +23,13 @@ CCCL_C_EXTERN_C_BEGIN
  * a digit pass works on tiles of onesweep_threads * onesweep_ipt items; the upsweep
  * gives each of about upsweep_blocks_per_cu * (compute units) workgroups a block of
  * consecutive tiles; inputs of at most single_tile_threads * single_tile_ipt items are
- * sorted by one workgroup. */
+ * sorted by one workgroup.  algorithm selects the digit pass:
+ *   CCCL_AMD_ALGO_PASSES   upsweep + bin scan + downsweep per digit (the default), or
+ *   CCCL_AMD_ALGO_ONESWEEP CUB's onesweep: one upfront histogram, then one kernel per
+ *                          digit with a decoupled look-back (inputs below 2^30 items;
+ *                          tile shapes of cccl_amd_onesweep_lookback_shapes). */
+#define CCCL_AMD_ALGO_PASSES 0
+#define CCCL_AMD_ALGO_ONESWEEP 1
 typedef struct cccl_amd_radix_sort_config_t
 {
   int32_t onesweep_threads;
@@ -32,7 +38,8 @@ typedef struct cccl_amd_radix_sort_config_t
   int32_t single_tile_threads; /* 0 disables the one-workgroup path */
   int32_t single_tile_ipt;
   int32_t upsweep_blocks_per_cu;
-  int32_t reserved[2];
+  int32_t algorithm;
+  int32_t reserved;
 } cccl_amd_radix_sort_config_t;
 
 /* Launch configuration of the batched segmented sort.  Segments are binned by size:
@@ -118,9 +125,12 @@ CCCL_C_API void cccl_amd_segmented_default_config(int key_bytes, int value_bytes
 /* 1 when the shapes in cfg were compiled for these widths and fit in LDS. */
 CCCL_C_API int cccl_amd_radix_sort_config_valid(const cccl_amd_radix_sort_config_t* cfg, int key_bytes, int value_bytes);
 
-/* Compiled onesweep / one-workgroup shapes: writes up to max (threads, ipt) pairs and
- * returns how many exist. */
+/* Compiled tile shapes: writes up to max (threads, ipt) pairs and returns how many exist.
+ * onesweep: digit-pass tiles of CCCL_AMD_ALGO_PASSES; onesweep_lookback: tiles of
+ * CCCL_AMD_ALGO_ONESWEEP and of the segmented sort's large segments; block_sort: the
+ * one-workgroup sort. */
 CCCL_C_API int cccl_amd_onesweep_shapes(int* threads, int* ipt, int max);
+CCCL_C_API int cccl_amd_onesweep_lookback_shapes(int* threads, int* ipt, int max);
 CCCL_C_API int cccl_amd_block_sort_shapes(int* threads, int* ipt, int max);
 
 /* Text describing the last failure of a cccl_amd_* call on this thread ("" if none). */

@@ -210,7 +210,8 @@ def test_empty_and_no_bits():
     assert db.selector == 0
 
 
-@pytest.mark.parametrize("shape", [(256, 8), (256, 16), (512, 8), (512, 12), (512, 16)])
+@pytest.mark.parametrize("shape", [(256, 8), (256, 16), (384, 16), (512, 8), (512, 12), (512, 16), (1024, 8),
+                                   (1024, 16)])
 @pytest.mark.parametrize("n", [8191, 300000])
 def test_every_onesweep_shape(shape, n):
     import cccl_amd as ca
@@ -227,6 +228,59 @@ def test_every_onesweep_shape(shape, n):
     assert sel == 1  # pointer form: the result is always in the second buffer
     ek, ev = oracle.sort(keys, vals)
     assert_same(to_host(ko, np.uint32), to_host(vo, np.uint32), ek, ev)
+
+
+@pytest.mark.parametrize("shape", [(256, 16), (512, 8), (512, 16), (1024, 16)])
+@pytest.mark.parametrize("case", ["u32_pairs", "f32_desc", "u64_window", "i16_keys"])
+@pytest.mark.parametrize("n", [8191, 300000, 1 << 21])
+def test_onesweep_lookback_algorithm(shape, case, n):
+    """CUB's onesweep (upfront histogram + decoupled look-back over atomic tile ids)
+    gives the oracle's result too."""
+    import cccl_amd as ca
+
+    rng = np.random.default_rng(n + shape[0] * shape[1])
+    begin, end, desc, vals = None, None, False, np.arange(n, dtype=np.uint32)
+    if case == "u32_pairs":
+        keys = random_keys(rng, np.uint32, n, "and3")
+    elif case == "f32_desc":
+        keys, desc = float_keys_with_specials(rng, np.float32, n), True
+    elif case == "u64_window":
+        keys, begin, end, desc = random_keys(rng, np.uint64, n), 0, 40, True
+    else:
+        keys, vals = random_keys(rng, np.int16, n), None
+    cfg = ca.make_config(shape[0], shape[1], single_tile_threads=0, algorithm=ca.ALGO_ONESWEEP)
+    if not ca.config_valid(cfg, keys.dtype.itemsize, 0 if vals is None else 4):
+        pytest.skip(f"a {shape} tile of {keys.dtype} keys and values does not fit in LDS")
+    tk = {np.uint32: torch.uint32, np.float32: torch.float32, np.uint64: torch.uint64, np.int16: torch.int16}
+    s = ca.Sorter(tk[keys.dtype.type], 0 if vals is None else 4, n, descending=desc, begin_bit=begin or 0,
+                  end_bit=end, config=cfg)
+    kd = to_dev(keys)
+    ko = torch.empty_like(kd)
+    vd = to_dev(vals) if vals is not None else None
+    vo = torch.empty_like(vd) if vals is not None else None
+    s.sort(kd, ko, vd, vo)
+    torch.cuda.synchronize()
+    ek, ev = oracle.sort(keys, vals, descending=desc, begin_bit=begin, end_bit=end)
+    assert_same(to_host(ko, keys.dtype), to_host(vo, np.uint32) if vals is not None else None, ek, ev)
+
+
+def test_algorithms_agree_large():
+    """2^24 u32 pairs: the three-kernel passes and the onesweep give identical output."""
+    import cccl_amd as ca
+
+    n = 1 << 24
+    g = torch.Generator(device="cuda").manual_seed(7)
+    kd = torch.randint(-(2**31), 2**31, (n,), dtype=torch.int32, device="cuda", generator=g).view(torch.uint32)
+    vd = torch.arange(n, dtype=torch.int32, device="cuda").view(torch.uint32)
+    outs = []
+    for algo in (ca.ALGO_PASSES, ca.ALGO_ONESWEEP):
+        cfg = ca.make_config(512, 16, algorithm=algo)
+        s = ca.Sorter(torch.uint32, 4, n, config=cfg)
+        ko, vo = torch.empty_like(kd), torch.empty_like(vd)
+        s.sort(kd, ko, vd, vo)
+        outs.append((ko, vo))
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
 
 
 @pytest.mark.parametrize("shape", [(256, 4), (256, 8), (256, 16), (512, 16)])
