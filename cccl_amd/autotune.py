@@ -246,6 +246,27 @@ def train_and_judge(records, train_frac=0.7, epochs=600, seed=0, device="cuda"):
     return net, out
 
 
+def load_timings(path) -> dict:
+    """Timings written by `stream`/`merge` (plain or gzip-compressed JSON)."""
+    import gzip
+
+    opener = gzip.open if str(path).endswith(".gz") else open
+    with opener(path, "rt") as f:
+        return json.load(f)
+
+
+def compact(data: dict) -> dict:
+    """The timings with numbers rounded (features to 6, times to 5 significant digits):
+    small enough to keep as a test fixture, unchanged for training and judging."""
+    recs = []
+    for r in data["records"]:
+        recs.append({"call": r["call"], "features": [float(f"{x:.6g}") for x in r["features"]],
+                     "feature_ms": float(f"{r['feature_ms']:.5g}"),
+                     "times_ms": [None if t is None else float(f"{t:.5g}") for t in r["times_ms"]],
+                     "valid": r["valid"], "num_segments": r["num_segments"]})
+    return {"arms": data["arms"], "records": recs}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -254,6 +275,11 @@ def main(argv=None) -> int:
     a.add_argument("--seed", type=int, default=5)
     a.add_argument("--log2-max", type=int, default=28)
     a.add_argument("--out", required=True)
+    a.add_argument("--start", type=int, default=0, help="first call of the stream to time")
+    a.add_argument("--count", type=int, default=None, help="number of calls to time (default: all)")
+    m = sub.add_parser("merge")
+    m.add_argument("parts", nargs="+")
+    m.add_argument("--out", required=True, help="output path; .gz writes compact gzip-compressed JSON")
     b = sub.add_parser("train")
     b.add_argument("--timings", required=True)
     b.add_argument("--weights", default=str(S.WEIGHTS_PATH))
@@ -262,9 +288,29 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.cmd == "stream":
         calls = make_stream(args.calls, args.seed, log2_max=args.log2_max)
-        run_stream(calls, args.out)
+        end = len(calls) if args.count is None else min(len(calls), args.start + args.count)
+        run_stream(calls[args.start:end], args.out)
         return 0
-    data = json.loads(open(args.timings).read())
+    if args.cmd == "merge":
+        parts = [load_timings(p) for p in args.parts]
+        if any(p["arms"] != parts[0]["arms"] for p in parts):
+            raise SystemExit("parts were timed with different arm lists")
+        records = sorted((r for p in parts for r in p["records"]), key=lambda r: r["call"]["id"])
+        ids = [r["call"]["id"] for r in records]
+        if len(set(ids)) != len(ids):
+            raise SystemExit("parts overlap")
+        merged = {"arms": parts[0]["arms"], "records": records}
+        if args.out.endswith(".gz"):
+            import gzip
+
+            with gzip.open(args.out, "wt") as f:
+                json.dump(compact(merged), f, separators=(",", ":"))
+        else:
+            with open(args.out, "w") as f:
+                json.dump(merged, f)
+        print(f"merged {len(records)} calls")
+        return 0
+    data = load_timings(args.timings)
     if data["arms"] != [x.name for x in S.ARMS]:
         raise SystemExit("timings were recorded for a different arm list")
     net, rep = train_and_judge(data["records"], epochs=args.epochs)

@@ -20,7 +20,7 @@ from pathlib import Path
 import torch
 
 from . import _bindings as B
-from .radix_sort import SegmentedSorter, SortError, Sorter, _stream_handle, make_config
+from .radix_sort import ALGO_ONESWEEP, ALGO_PASSES, SegmentedSorter, SortError, Sorter, _stream_handle, make_config
 
 NUM_FEATURES = 16
 WEIGHTS_PATH = Path(__file__).resolve().parent / "selector_weights.json"
@@ -36,10 +36,11 @@ class Arm:
     block_classes: tuple[tuple[int, int], ...] = ()
     large: tuple[int, int] = (512, 16)
     strategy: int = 0
+    algorithm: int = ALGO_PASSES
 
     def plain_config(self) -> B.radix_sort_config:
         return make_config(self.onesweep[0], self.onesweep[1], single_tile_threads=self.single_tile[0],
-                           single_tile_ipt=self.single_tile[1])
+                           single_tile_ipt=self.single_tile[1], algorithm=self.algorithm)
 
     def segmented_config(self) -> B.segmented_config:
         c = B.segmented_config()
@@ -51,12 +52,19 @@ class Arm:
         return c
 
 
+PASS_SHAPES = [(256, 8), (256, 16), (384, 16), (512, 8), (512, 12), (512, 16), (1024, 8), (1024, 16)]
+LOOKBACK_SHAPES = [(256, 16), (512, 8), (512, 16), (1024, 16)]
+
+
 def _plain_arms() -> list[Arm]:
     arms = []
-    for os_shape in [(256, 8), (256, 16), (512, 8), (512, 12), (512, 16)]:
+    for os_shape in PASS_SHAPES:
         for st in [(0, 0), (256, 16), (512, 16)]:
             arms.append(Arm(f"plain os{os_shape[0]}x{os_shape[1]} st{st[0] * st[1]}", "plain", onesweep=os_shape,
                             single_tile=st))
+    for os_shape in LOOKBACK_SHAPES:
+        arms.append(Arm(f"plain lookback{os_shape[0]}x{os_shape[1]} st4096", "plain", onesweep=os_shape,
+                        single_tile=(256, 16), algorithm=ALGO_ONESWEEP))
     return arms
 
 
@@ -69,7 +77,7 @@ def _segmented_arms() -> list[Arm]:
         "c0": (),
     }
     for cname, classes in class_sets.items():
-        for large in [(256, 16), (512, 8), (512, 16)]:
+        for large in LOOKBACK_SHAPES:
             arms.append(Arm(f"seg {cname} L{large[0]}x{large[1]}", "segmented", block_classes=classes, large=large))
     for large in [(256, 16), (512, 8)]:
         arms.append(Arm(f"seg per-segment-pass {large[0]}x{large[1]}", "segmented", large=large, strategy=1))
