@@ -1,22 +1,39 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X radix sort: 2^28 uint32 keys + uint32 values (value = index),
-ascending, pointer form (input untouched, so every step sorts the same input).
+"""Benchmark of the MI355X radix sort.
 
-One step = one full SortPairs over resident HBM buffers (histogram + 4 onesweep
-passes).  Prints one JSON line: Gkeys/s of the uniform-key workload, plus per-entropy
-rates, the onesweep kernel's HBM roofline fraction (HIP events around each launch on
-the sort's stream), the 1M-key keys-only latency, and the reference's Thrust OpenMP
-sort timed on a bounded sample on this host's cores.
+Headline workload: SortPairs of 2^28 uint32 keys + uint32 values (value = index),
+ascending, pointer form (input untouched, so every step sorts the same input), over
+HBM-resident buffers.  One step = one full sort (init, then per 8-bit digit pass an
+upsweep, a bin scan and a downsweep).  Prints one JSON line with:
+
+* value: Gkeys/s of the uniform-key workload; rates and parity properties at six key
+  entropies;
+* roofline: the downsweep (the dominant kernel: reads and writes every key and value
+  once per pass) -- algorithmic bytes 2*N*(K+V) per launch over its average launch
+  time from HIP events on the sort's stream, against 8 TB/s; the copy bandwidth this
+  GPU reaches in the same run is reported beside it; traffic from the committed
+  rocprofv3 counter summary of the same kernel when present;
+* the reference's onesweep design (upfront histogram + decoupled look-back) on the same
+  input, for comparison;
+* the 1M-key keys-only latency, the 128M float (descending, +-0 / +-inf) and 128M
+  uint64 [0,40) descending pair sorts;
+* the batched segmented sort of 100,000 lognormal segments (2^28 items): autotuned
+  size-binned batch, its hipGraph replay, CUB's one-workgroup-per-segment-per-pass
+  schedule, and a per-segment loop of plain sorts (timed on a random sample of
+  segments, scaled by the segment count);
+* the learned config selector on held-out calls of the 2,000-call stream, timed live
+  against the fixed default configuration and the exhaustive best;
+* the reference's Thrust OpenMP stable_sort_by_key on a bounded sample, on this host.
 """
 
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import statistics
 import sys
-import time
 from pathlib import Path
 
 REPO = Path(__file__).resolve().parent
@@ -24,6 +41,7 @@ sys.path.insert(0, str(REPO))
 
 HBM_PEAK_GBS = 8000.0
 ENTROPIES = ["uniform", "and2", "and3", "and4", "and5", "equal"]
+PMC_SUMMARY = REPO / "profiles" / "downsweep_u32_pairs_256M_pmc.json"
 
 
 def gen_keys(torch, n: int, entropy: str, seed: int, device):
@@ -55,24 +73,29 @@ def time_steps(torch, fn, steps: int, warmup: int) -> float:
     return start.elapsed_time(stop) / steps
 
 
-def check_sorted_pairs(torch, kin, kout, vout) -> bool:
-    """Sorted, stable, a permutation, and every key matches its value's input key."""
+def check_sorted(torch, kin, kout, vout, order_key, descending=False) -> bool:
+    """Sorted by order_key (a monotone int64 image of the key), stable (equal keys keep
+    ascending values), a permutation, and every key is its value's input key (bitwise)."""
     n = kin.numel()
-    k = kout.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
-    ok = bool((k[1:] >= k[:-1]).all())
+    k = order_key(kout)
+    ok = bool((k[1:] <= k[:-1]).all()) if descending else bool((k[1:] >= k[:-1]).all())
     eq = k[1:] == k[:-1]
     v = vout.view(torch.int32).to(torch.int64)
     ok &= bool((v[1:][eq] > v[:-1][eq]).all())
     ok &= bool((torch.sort(v).values == torch.arange(n, device=v.device)).all())
-    ok &= bool((kin.view(torch.int32)[v] == kout.view(torch.int32)).all())
+    w = torch.int64 if kin.element_size() == 8 else torch.int32
+    ok &= bool((kin.view(w)[v] == kout.view(w)).all())
     return ok
 
 
-def kernel_times(torch, ca, fn):
+def u32_order(torch):
+    return lambda k: k.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+
+
+def kernel_times(torch, fn):
     """Per-launch durations (ms) by kind, from HIP events the library records on the
     sort's own stream around each launch."""
     from cccl_amd import _bindings as B
-    import ctypes
 
     L = B.lib()
     torch.cuda.synchronize()
@@ -81,13 +104,26 @@ def kernel_times(torch, ca, fn):
     fn()
     torch.cuda.synchronize()
     L.cccl_amd_profile_enable(0)
-    kinds = (ctypes.c_int * 256)()
-    ms = (ctypes.c_float * 256)()
-    cnt = min(L.cccl_amd_profile_read(kinds, ms, 256), 256)
+    kinds = (ctypes.c_int * 512)()
+    ms = (ctypes.c_float * 512)()
+    cnt = min(L.cccl_amd_profile_read(kinds, ms, 512), 512)
     out: dict[int, list[float]] = {}
     for i in range(cnt):
         out.setdefault(kinds[i], []).append(ms[i])
     return out
+
+
+def avg(xs):
+    return sum(xs) / len(xs) if xs else None
+
+
+def copy_peak_gbs(torch, dev) -> float:
+    """Copy bandwidth this GPU reaches in this run: 1 GiB device-to-device copy."""
+    src = torch.empty(1 << 28, dtype=torch.int32, device=dev).fill_(1)
+    dst = torch.empty_like(src)
+    ms = time_steps(torch, lambda: dst.copy_(src), 10, 2)
+    del src, dst
+    return 2 * (1 << 30) / (ms * 1e-3) / 1e9
 
 
 def cpu_baseline(keys_host_u32, threads: int, runs: int = 3):
@@ -108,6 +144,188 @@ def cpu_baseline(keys_host_u32, threads: int, runs: int = 3):
                       f"wall time of the call only"}
 
 
+def leg_wide_keys(torch, ca, dev, steps):
+    """128M float32 descending with +-0 / +-inf, and 128M uint64 bits [0,40) descending,
+    both with int32 values = index."""
+    from cccl_amd import workloads as W
+
+    n = 1 << 27
+    vin = torch.arange(n, dtype=torch.int32, device=dev)
+    vout = torch.empty_like(vin)
+    out = {}
+    kf = W.float_keys_with_specials(n, 43, dev)
+    ko = torch.empty_like(kf)
+    s = ca.Sorter(torch.float32, 4, n, descending=True)
+    ms = time_steps(torch, lambda: s.sort(kf, ko, vin, vout), steps, 1)
+
+    def f32_order(k):
+        # descending float order with -0 == +0: the float value itself (no NaN here)
+        return k.to(torch.float64)
+
+    out["f32_desc_128M"] = {"gkeys_per_s": round(n / ms / 1e6, 3), "ms": round(ms, 4),
+                            "parity_properties_ok": check_sorted(torch, kf, ko, vout, f32_order, descending=True)}
+    del kf, ko, s
+    ku = W.int_keys(n, 8, "uniform", 44, dev).view(torch.uint64)
+    ko = torch.empty_like(ku)
+    s = ca.Sorter(torch.uint64, 4, n, descending=True, begin_bit=0, end_bit=40)
+    ms = time_steps(torch, lambda: s.sort(ku, ko, vin, vout), steps, 1)
+    mask = (1 << 40) - 1
+    out["u64_bits0_40_desc_128M"] = {
+        "gkeys_per_s": round(n / ms / 1e6, 3), "ms": round(ms, 4), "passes": 5,
+        "parity_properties_ok": check_sorted(torch, ku, ko, vout, lambda k: k.view(torch.int64) & mask,
+                                             descending=True)}
+    return out
+
+
+def leg_segmented(torch, ca, dev, steps):
+    """100,000 lognormal segments (median 512, sigma 2, clipped to [1, 2^21], rescaled to
+    2^28 items), uniform uint32 keys, uint32 values = index, int32 offsets."""
+    import numpy as np
+
+    from cccl_amd import _bindings as B
+    from cccl_amd import selector as S
+    from cccl_amd import workloads as W
+    from cccl_amd.graph import SortGraph
+    from cccl_amd.radix_sort import _stream_handle, key_type_of
+
+    n, nseg = 1 << 28, 100000
+    sizes = W.lognormal_sizes(nseg, n, 42)
+    begins, ends = W.offsets_from_sizes(sizes, dev)
+    kin = gen_keys(torch, n, "uniform", 45, dev)
+    vin = torch.arange(n, dtype=torch.int32, device=dev).view(torch.uint32)
+    kout, vout = torch.empty_like(kin), torch.empty_like(vin)
+
+    def run(arm):
+        s = S.make_sorter(arm, torch.uint32, 4, n, False, 0, 32, nseg)
+        return s, (lambda: s.sort(kin, kout, begins, ends, vin, vout))
+
+    # the autotuner's pick for this workload: every valid segmented arm once
+    tried = {}
+    for i, arm in enumerate(S.ARMS):
+        if arm.kind == "segmented" and S.arm_valid(arm, 4, 4, True):
+            s, fn = run(arm)
+            tried[i] = time_steps(torch, fn, 1, 1)
+            del s
+    best = min(tried, key=tried.get)
+    s, fn = run(S.ARMS[best])
+    ms_batch = time_steps(torch, fn, steps, 1)
+    times = kernel_times(torch, fn)
+
+    # parity: every segment sorted and stable, values stay in their segment, permutation
+    seg = torch.repeat_interleave(torch.arange(nseg, device=dev), torch.from_numpy(sizes).to(dev))
+    k = kout.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    same = seg[1:] == seg[:-1]
+    v = vout.view(torch.int32).to(torch.int64)
+    ok = bool((k[1:][same] >= k[:-1][same]).all())
+    eq = same & (k[1:] == k[:-1])
+    ok &= bool((v[1:][eq] > v[:-1][eq]).all())
+    ok &= bool(((v >= begins.to(torch.int64)[seg]) & (v < ends.to(torch.int64)[seg])).all())
+    ok &= bool((torch.sort(v).values == torch.arange(n, device=dev)).all())
+    ok &= bool((kin.view(torch.int32)[v] == kout.view(torch.int32)).all())
+    ref_k, ref_v = kout.clone(), vout.clone()
+    kout.zero_()
+    vout.zero_()
+
+    # the same calls captured in a hipGraph and replayed
+    side = torch.cuda.Stream()
+    g = SortGraph()
+    with torch.cuda.stream(side):
+        with g.capture(side):
+            s.sort(kin, kout, begins, ends, vin, vout, stream=side)
+    side.synchronize()
+
+    def replay():
+        g.launch(side)
+
+    for _ in range(2):
+        replay()
+    side.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(side)
+    for _ in range(steps):
+        replay()
+    b.record(side)
+    side.synchronize()
+    ms_graph = a.elapsed_time(b) / steps
+    graph_ok = bool(torch.equal(kout, ref_k) and torch.equal(vout, ref_v))  # replays give the eager result
+    del g, s
+
+    # CUB's DeviceSegmentedRadixSort schedule: one workgroup per segment per pass
+    cub_arm = S.FIXED_DEFAULT["segmented"]
+    s, fn = run(S.ARMS[cub_arm])
+    ms_cub = time_steps(torch, fn, max(2, steps // 2), 1)
+    del s
+
+    # a per-segment loop of plain default-configuration sorts, on a random sample
+    rng = np.random.default_rng(46)
+    sample = rng.choice(nseg, size=2000, replace=False)
+    big = int(sizes.max())
+    probe = ctypes.c_size_t(0)
+    sel = ctypes.c_int(0)
+    L = B.lib()
+    kt = key_type_of(torch.uint32)
+    L.cccl_amd_radix_sort(None, ctypes.byref(probe), None, None, None, None, ctypes.byref(sel), kt, 4, big, 0, 32, 0,
+                          0, None, None)
+    temp = torch.empty(max(probe.value, 1), dtype=torch.uint8, device=dev)
+    stream = _stream_handle(None)
+    b_host = (np.cumsum(sizes) - sizes).astype(np.int64)
+
+    def loop():
+        for j in sample:
+            off, cnt = int(b_host[j]), int(sizes[j])
+            nb = ctypes.c_size_t(temp.numel())
+            sel.value = 0
+            L.cccl_amd_radix_sort(temp.data_ptr(), ctypes.byref(nb), kin.data_ptr() + 4 * off,
+                                  kout.data_ptr() + 4 * off, vin.data_ptr() + 4 * off, vout.data_ptr() + 4 * off,
+                                  ctypes.byref(sel), kt, 4, cnt, 0, 32, 0, 0, None, stream)
+
+    ms_loop_sample = time_steps(torch, loop, 1, 1)
+    ms_loop = ms_loop_sample * nseg / sample.size
+    return {
+        "workload": f"{nseg} segments, lognormal sizes (median 512, sigma 2.0, clipped to [1, 2^21], rescaled to "
+                    f"{n} items; largest {big}), uint32 keys + uint32 values, int32 offsets",
+        "autotuned_config": S.ARMS[best].name,
+        "batch_ms": round(ms_batch, 4), "batch_gkeys_per_s": round(n / ms_batch / 1e6, 3),
+        "hipgraph_replay_ms": round(ms_graph, 4), "hipgraph_replay_ok": graph_ok,
+        "cub_schedule_ms": round(ms_cub, 4), "cub_schedule_config": S.ARMS[cub_arm].name,
+        "per_segment_loop_ms_scaled": round(ms_loop, 2),
+        "per_segment_loop_sample": f"{sample.size} random segments timed ({ms_loop_sample:.2f} ms), x {nseg / sample.size:.0f}",
+        "speedup_vs_cub_schedule": round(ms_cub / min(ms_batch, ms_graph), 3),
+        "speedup_vs_per_segment_loop": round(ms_loop / min(ms_batch, ms_graph), 2),
+        "kernel_ms_by_kind": {str(k): round(sum(v), 4) for k, v in times.items()},
+        "parity_properties_ok": ok,
+    }
+
+
+def leg_selector(torch, dev, calls: int):
+    """Held-out calls of the 2,000-call stream (the last 30%, every k-th): the shipped
+    network picks an arm from features computed live; the pick, the fixed default arm
+    and every valid arm are timed live."""
+    from cccl_amd import autotune as A
+    from cccl_amd import selector as S
+
+    stream = A.make_stream(2000, 5)
+    held = stream[1400:]
+    step = max(1, len(held) // max(calls, 1))
+    picked = held[::step][:calls]
+    net = S.MLPSelector.load()
+    buf = A.Buffers(max(c["n"] for c in picked), dev)
+    recs, choice = [], []
+    for c in picked:
+        rec = A.measure_call(c, buf)
+        x = torch.tensor([rec["features"]], dtype=torch.float32, device=dev)
+        valid = torch.tensor([rec["valid"]], device=dev)
+        choice.append(int(net.choose(x, valid)[0]))
+        recs.append(rec)
+    j = A.judge(recs, choice)
+    return {"calls": len(recs), "held_out_of": "calls 1400..1999 of the stream (seed 5)",
+            "speedup_vs_fixed_default": round(j["speedup_vs_default"], 3),
+            "regret_vs_exhaustive": round(j["regret_vs_exhaustive"], 4),
+            "regret_excl_feature_time": round(j["regret_vs_exhaustive_excl_features"], 4),
+            "selected_ms": round(j["selected_ms"], 3), "feature_ms": round(j["feature_ms"], 3),
+            "default_ms": round(j["default_ms"], 3), "exhaustive_ms": round(j["exhaustive_ms"], 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +334,8 @@ def main() -> int:
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--entropies", default=",".join(ENTROPIES))
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline workload only")
+    ap.add_argument("--selector-calls", type=int, default=100)
     ap.add_argument("--cpu-sample-log2", type=int, default=25)
     ap.add_argument("--pmc-workload", action="store_true",
                     help="counter runs: one 2 GiB device copy (reference bytes), then --steps sorts only")
@@ -137,7 +357,7 @@ def main() -> int:
     kout = torch.empty(n, dtype=torch.int32, device=dev).view(torch.uint32)
     vout = torch.empty(n, dtype=torch.int32, device=dev).view(torch.uint32)
     sorter = ca.Sorter(torch.uint32, 4, n)
-    cfg = sorter.config if sorter.config is not None else ca.default_config(4, 4, n)
+    cfg = ca.default_config(4, 4, n)
 
     if args.pmc_workload:
         # A plain copy of known size first: its FETCH_SIZE / WRITE_SIZE give the scale
@@ -154,37 +374,63 @@ def main() -> int:
         print(json.dumps({"pmc_workload": "copy 2 GiB int64, then sorts", "sorts": args.steps, "n": n}), flush=True)
         return 0
 
-    rates = {}
-    parity = {}
-    head = None
+    rates, parity, head, ktimes = {}, {}, None, {}
     for ent in entropies:
         kin = gen_keys(torch, n, ent, 42, dev)
         fn = lambda: sorter.sort(kin, kout, vin, vout)  # noqa: E731
         steps = args.steps if ent == "uniform" else max(2, args.steps // 2)
         ms = time_steps(torch, fn, steps, args.warmup if ent == "uniform" else 1)
         rates[ent] = round(n / (ms * 1e-3) / 1e9, 3)
-        parity[ent] = check_sorted_pairs(torch, kin, kout, vout)
+        parity[ent] = check_sorted(torch, kin, kout, vout, u32_order(torch))
         if ent == "uniform":
             head = (ms, steps)
-            times = kernel_times(torch, ca, fn)
-            ktimes = {k: v for k, v in times.items()}
+            ktimes = kernel_times(torch, fn)
             cpu_sample = kin[: 1 << min(args.cpu_sample_log2, args.log2n)].view(torch.int32).cpu().numpy()
+            # the reference's onesweep design on the same input
+            os_cfg = ca.make_config(1024, 16, algorithm=ca.ALGO_ONESWEEP)
+            s_os = ca.Sorter(torch.uint32, 4, n, config=os_cfg)
+            fn_os = lambda: s_os.sort(kin, kout, vin, vout)  # noqa: E731
+            ms_os = time_steps(torch, fn_os, max(2, args.steps // 2), 1)
+            t_os = kernel_times(torch, fn_os)
+            onesweep_ok = check_sorted(torch, kin, kout, vout, u32_order(torch))
+            del s_os
         del kin
     if head is None:
         raise SystemExit("the uniform workload must be part of --entropies")
     ms, steps = head
 
-    onesweep = ktimes.get(2, [])
-    hist = ktimes.get(1, [])
-    avg_os = sum(onesweep) / len(onesweep)
+    down, up, spine = ktimes.get(2, []), ktimes.get(1, []), ktimes.get(8, [])
+    avg_down = avg(down)
     alg_pass_bytes = 16 * n  # read + write of 4-byte keys and 4-byte values
-    achieved = alg_pass_bytes / (avg_os * 1e-3) / 1e9
-    e2e_bytes = 68 * n  # 4 passes x 16N + one 4N histogram read
+    achieved = alg_pass_bytes / (avg_down * 1e-3) / 1e9
+    peak_copy = copy_peak_gbs(torch, dev)
+    traffic = None
+    traffic_note = "no committed counter summary"
+    if PMC_SUMMARY.exists():
+        pmc = json.loads(PMC_SUMMARY.read_text())
+        if pmc.get("n") == n and pmc.get("config") == [cfg.onesweep_threads, cfg.onesweep_ipt]:
+            traffic = pmc["downsweep_bytes_per_launch"]
+            traffic_note = f"rocprofv3 FETCH_SIZE + WRITE_SIZE, {PMC_SUMMARY.relative_to(REPO)}"
+        else:
+            traffic_note = "committed counter summary is for another configuration"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "onesweep_kernel<u32,u32> (one 8-bit digit pass)",
-                "kernel_avg_ms": round(avg_os, 4), "launches_measured": len(onesweep),
-                "alg_bytes_per_launch": alg_pass_bytes}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "onesweep_kernel<u32,u32> downsweep (one 8-bit digit pass: rank + scatter)",
+                "kernel_avg_ms": round(avg_down, 4), "launches_measured": len(down),
+                "alg_bytes_per_launch": alg_pass_bytes, "traffic_source": traffic_note,
+                "copy_peak_measured_GBs": round(peak_copy, 1),
+                "frac_of_measured_copy_peak": round(achieved / peak_copy, 4)}
+    # algorithmic bytes of the sort (4 passes x 16N + one 4N histogram read), and the
+    # bytes this design moves (4 passes x (16N scatter + 4N upsweep read))
+    e2e_bytes = 68 * n
+    moved_bytes = 4 * (16 + 4) * n
+    passes = {"upsweep_ms": round(avg(up), 4), "spine_ms": round(avg(spine), 4), "downsweep_ms": round(avg_down, 4),
+              "pass_ms": round(avg(up) + avg(spine) + avg_down, 4),
+              "upsweep_read_GBs": round(4 * n / (avg(up) * 1e-3) / 1e9, 1)}
+    os_pass = t_os.get(6, [])
+    onesweep = {"config": "onesweep 1024x16, upfront histogram + decoupled look-back", "ms": round(ms_os, 4),
+                "gkeys_per_s": round(n / ms_os / 1e6, 3), "pass_ms": round(avg(os_pass), 4) if os_pass else None,
+                "histogram_ms": round(sum(t_os.get(5, [])), 4), "parity_properties_ok": onesweep_ok}
 
     # 1M uint32 keys-only ascending: the latency-bound configuration
     n1 = 1 << 20
@@ -192,6 +438,18 @@ def main() -> int:
     k1o = torch.empty_like(k1)
     s1 = ca.Sorter(torch.uint32, 0, n1)
     ms1 = time_steps(torch, lambda: s1.sort(k1, k1o), 50, 5)
+    del s1, k1, k1o
+
+    extra = {}
+    if not args.quick:
+        del vin, kout, vout, sorter
+        torch.cuda.empty_cache()
+        extra["wide_keys"] = leg_wide_keys(torch, ca, dev, max(2, args.steps // 2))
+        torch.cuda.empty_cache()
+        extra["segmented_batch"] = leg_segmented(torch, ca, dev, max(2, args.steps // 2))
+        torch.cuda.empty_cache()
+        if args.selector_calls > 0:
+            extra["learned_selector"] = leg_selector(torch, dev, args.selector_calls)
 
     cpu = None
     if not args.skip_cpu:
@@ -215,15 +473,20 @@ def main() -> int:
         "data": "synthetic (torch Philox, seed 42)",
         "config": {"workload": f"SortPairs ascending, {n} uint32 keys (uniform 32-bit) + uint32 values "
                                f"(value = index), bits [0,32), pointer form",
-                   "onesweep_threads": cfg.onesweep_threads, "onesweep_ipt": cfg.onesweep_ipt, "radix_bits": 8},
+                   "tile_threads": cfg.onesweep_threads, "tile_ipt": cfg.onesweep_ipt, "radix_bits": 8,
+                   "algorithm": "upsweep + bin scan + downsweep per pass"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "e2e_hbm_frac": round(e2e_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "histogram_ms": round(sum(hist) / max(len(hist), 1), 4),
+        "e2e_alg_bytes": e2e_bytes,
+        "e2e_bytes_moved_frac": round(moved_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "pass_breakdown": passes,
+        "onesweep_lookback_design": onesweep,
         "gkeys_per_s_by_entropy": rates,
         "parity_properties_ok": parity,
         "sortkeys_1m_u32_us": round(ms1 * 1e3, 2),
         "sortkeys_1m_u32_gkeys_per_s": round(n1 / (ms1 * 1e-3) / 1e9, 3),
+        **extra,
     }
     print(json.dumps(out), flush=True)
     return 0
