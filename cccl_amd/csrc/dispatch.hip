@@ -556,8 +556,7 @@ hipError_t radix_sort(
   // Blocks of consecutive tiles for the upsweep: about upsweep_blocks_per_cu per compute
   // unit, at most kMaxTilesPerBlock tiles each (block sums stay below 2^32).
   const uint64_t want_blocks = uint64_t(device_cu_count()) * uint64_t(std::max(1, cfg.upsweep_blocks_per_cu));
-  const uint64_t tpb =
-    std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(tiles, want_blocks), kMaxTilesPerBlock));
+  const uint64_t tpb = std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(tiles, want_blocks), kMaxTilesPerBlock));
   const uint64_t blocks      = ceil_div(tiles, tpb);
   const bool need_tmp        = !pr.overwrite && passes > 1;
 
@@ -622,6 +621,13 @@ hipError_t radix_sort(
     upp.float_mode      = float_mode;
     upp.shift           = bit;
     upp.num_bits        = num_bits;
+    const bool fused    = blocks <= kMaxFusedSpineBlocks;
+    if (fused)
+    {
+      upp.done         = d_done + pass;
+      upp.block_prefix = d_block_prefix;
+      upp.digit_start  = d_digit_start;
+    }
     if (hipError_t e = launch_kernel(up->fn, blocks, kUpsweepThreads, size_t(tpb) * kRadix * sizeof(uint32_t), stream,
                                      &upp, 1);
         e != hipSuccess)
@@ -629,11 +635,15 @@ hipError_t radix_sort(
       return e;
     }
 
-    SpineParams sp{d_block_sum, d_block_prefix, d_totals, d_digit_start, d_done + pass, uint32_t(blocks)};
-    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&spine_kernel), kRadix, kRadix, 0, stream, &sp, 8);
-        e != hipSuccess)
+    if (!fused)
     {
-      return e;
+      SpineParams sp{d_block_sum, d_block_prefix, d_totals, d_digit_start, d_done + pass, uint32_t(blocks)};
+      if (hipError_t e =
+            launch_kernel(reinterpret_cast<const void*>(&spine_kernel), kRadix, kRadix, 0, stream, &sp, 8);
+          e != hipSuccess)
+      {
+        return e;
+      }
     }
 
     OnesweepParams op{};

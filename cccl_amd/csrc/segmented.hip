@@ -2,10 +2,11 @@
 //
 // Launch sequence (stream-ordered, no host synchronisation, no memset nodes, so a
 // caller can capture it in a hipGraph and replay it):
-//   seg_init      zero the class counters, tile counters and the first look-back buffer
+//   seg_init      zero the class counters and tile counters
 //   seg_classify  bin every segment by size into the one-workgroup classes or the large
 //                 class; large segments get a contiguous range of onesweep tiles
-//   seg_prep      per large segment: clear its digit counts, map its tiles to it
+//   seg_prep      per large segment: clear its digit counts and its tiles' first-pass
+//                 look-back words, map its tiles to it
 //   seg_histogram per large tile: digit counts of every pass into its segment's counts
 //   block_sort    one launch per one-workgroup class, one workgroup per segment
 //   onesweep      one batched launch per digit pass over every large segment's tiles
@@ -288,8 +289,8 @@ hipError_t segmented_radix_sort(
   void* val_final = val_seq[(passes - 1) & 1];
 
   {
-    SegInitParams ip{d_counters, uint32_t(num_counters), d_lb[0], tile_bound * kRadix};
-    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(ceil_div<uint64_t>(ip.num_lookback, 256), 2048));
+    SegInitParams ip{d_counters, uint32_t(num_counters)};
+    const uint64_t grid = ceil_div<uint64_t>(uint64_t(num_counters), 256);
     if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&seg_init_kernel), grid, 256, 0, stream, &ip, 4);
         e != hipSuccess)
     {
@@ -321,7 +322,8 @@ hipError_t segmented_radix_sort(
   }
   if (large_bound > 0)
   {
-    SegPrepParams pp{d_counters, d_large_range, d_large_first, d_tile_slot, d_seg_bins, uint32_t(large_tile), passes};
+    SegPrepParams pp{d_counters, d_large_range, d_large_first, d_tile_slot, d_seg_bins, d_lb[0], uint32_t(large_tile),
+                     passes};
     if (hipError_t e =
           launch_kernel(reinterpret_cast<const void*>(&seg_prep_kernel), large_bound, 256, 0, stream, &pp, 4);
         e != hipSuccess)

@@ -89,7 +89,15 @@ struct UpsweepParams
   int float_mode;
   int shift;
   int num_bits;
+  // Spine folded into the upsweep (few blocks): the workgroup that finishes last scans
+  // the block sums itself; null done = a separate spine_kernel launch does it.
+  uint32_t* done; // zero on entry
+  uint64_t* block_prefix;
+  uint64_t* digit_start;
 };
+
+// Largest block count for which the upsweep's last workgroup does the spine's work.
+constexpr uint32_t kMaxFusedSpineBlocks = 64;
 
 // Largest number of tiles one upsweep workgroup counts (its LDS holds one row of
 // counters per tile).
@@ -225,6 +233,48 @@ __device__ __forceinline__ void upsweep_block(const UpsweepParams& p, uint32_t* 
       run += c;
     }
     p.block_sum[uint64_t(tid) * p.num_blocks + b] = run;
+  }
+  if (p.done == nullptr)
+  {
+    return;
+  }
+  // Folded spine.  The full barrier waits for every wave's block_sum stores, so thread
+  // 0's release covers them; the workgroup whose increment completes the count sees
+  // every block's sums.
+  __shared__ uint64_t s_wave[kRadix / 64];
+  __shared__ uint32_t s_last;
+  __syncthreads();
+  if (tid == 0)
+  {
+    const uint32_t prev = __hip_atomic_fetch_add(p.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last              = prev + 1u == p.num_blocks ? 1u : 0u;
+  }
+  __syncthreads();
+  if (s_last == 0u) // block-uniform
+  {
+    return;
+  }
+  // Every wave acquires at device scope (its stale cached lines are dropped), so the
+  // plain loads below see every block's sums and can all be in flight at once.
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  uint64_t total = 0;
+  if (tid < uint32_t(kRadix))
+  {
+    const uint32_t* col = p.block_sum + uint64_t(tid) * p.num_blocks;
+    uint64_t* pre       = p.block_prefix + uint64_t(tid) * p.num_blocks;
+#pragma unroll 8
+    for (uint32_t j = 0; j < p.num_blocks; ++j)
+    {
+      const uint32_t c = col[j];
+      pre[j]           = total;
+      total += c;
+    }
+  }
+  uint64_t all           = 0;
+  const uint64_t start   = radix_exclusive_scan<uint64_t>(total, s_wave, all);
+  if (tid < uint32_t(kRadix))
+  {
+    p.digit_start[tid] = start;
   }
 }
 
@@ -1129,21 +1179,14 @@ struct SegInitParams
 {
   uint32_t* counters; // kSegCounters + passes tile counters
   uint32_t num_counters;
-  uint32_t* lookback;
-  uint64_t num_lookback;
 };
 
 static __global__ void __launch_bounds__(256) seg_init_kernel(SegInitParams p)
 {
-  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-  const uint64_t i0     = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i0 < p.num_counters)
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p.num_counters)
   {
-    p.counters[i0] = 0;
-  }
-  for (uint64_t i = i0; i < p.num_lookback; i += stride)
-  {
-    p.lookback[i] = 0;
+    p.counters[i] = 0;
   }
 }
 
@@ -1211,12 +1254,14 @@ struct SegPrepParams
   const uint32_t* large_first_tile;
   uint32_t* tile_slot;
   uint32_t* seg_bins; // [slot][passes][kRadix]
+  uint32_t* lookback; // look-back buffer of the first pass: cleared for the slot's tiles
   uint32_t large_tile;
   int num_passes;
 };
 
 // One workgroup per large-segment slot (the grid is an upper bound on their number):
-// clears the slot's digit counts and writes the slot id of each of its tiles.
+// clears the slot's digit counts and its tiles' first-pass look-back words, and writes
+// the slot id of each of its tiles.
 static __global__ void __launch_bounds__(256) seg_prep_kernel(SegPrepParams p)
 {
   const uint32_t slot = blockIdx.x;
@@ -1235,6 +1280,10 @@ static __global__ void __launch_bounds__(256) seg_prep_kernel(SegPrepParams p)
   for (uint32_t t = threadIdx.x; t < tiles; t += blockDim.x)
   {
     p.tile_slot[first + t] = slot;
+  }
+  for (uint64_t i = threadIdx.x; i < uint64_t(tiles) * kRadix; i += blockDim.x)
+  {
+    p.lookback[uint64_t(first) * kRadix + i] = 0u;
   }
 }
 
