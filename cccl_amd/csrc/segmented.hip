@@ -8,6 +8,7 @@
 //   seg_prep      per large segment: clear its digit counts and its tiles' first-pass
 //                 look-back words, map its tiles to it
 //   seg_histogram per large tile: digit counts of every pass into its segment's counts
+//   seg_starts    per large segment: the counts become digit starts within the segment
 //   block_sort    one launch per one-workgroup class, one workgroup per segment
 //   onesweep      one batched launch per digit pass over every large segment's tiles
 // Pass count and final-buffer rules follow
@@ -344,6 +345,13 @@ hipError_t segmented_radix_sort(
     hp.end_bit         = pr.end_bit;
     hp.num_passes      = passes;
     if (hipError_t e = launch_kernel(tab->seg_histogram, tile_bound, kHistThreads, 0, stream, &hp, 5); e != hipSuccess)
+    {
+      return e;
+    }
+    SegStartsParams stp{d_counters, d_seg_bins, passes};
+    if (hipError_t e =
+          launch_kernel(reinterpret_cast<const void*>(&seg_starts_kernel), large_bound, kRadix, 0, stream, &stp, 5);
+        e != hipSuccess)
     {
       return e;
     }

@@ -389,7 +389,7 @@ struct OnesweepParams
   const uint32_t* tile_slot; // segment slot of each tile
   const uint32_t* slot_first_tile; // first tile of each slot
   const int64_t* slot_range; // [slot][2] = begin, end item offsets
-  const uint32_t* seg_bins; // [slot][num_passes][kRadix] digit counts
+  const uint32_t* seg_bins; // [slot][num_passes][kRadix] digit starts within the segment
   const uint32_t* total_tiles; // number of real tiles (device)
   int pass;
   int num_passes;
@@ -486,7 +486,6 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
   uint64_t* s_base   = reinterpret_cast<uint64_t*>(smem + L::kOffBase);
   uint32_t* s_base32 = reinterpret_cast<uint32_t*>(smem + L::kOffBase); // low words of s_base
   uint32_t* s_misc   = reinterpret_cast<uint32_t*>(smem + L::kOffMisc);
-  uint64_t* s_scan64 = reinterpret_cast<uint64_t*>(s_misc + 8);
   UK* s_keys         = reinterpret_cast<UK*>(smem + L::kOffKeys);
   VS* s_vals         = reinterpret_cast<VS*>(smem + L::kOffVals);
 
@@ -553,6 +552,7 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
   uint32_t first_tile = 0; // tile whose look-back needs no predecessor
   uint32_t slot       = 0;
   uint64_t seg_begin  = 0;
+  uint32_t seg_start  = 0; // SEG, thread d < kRadix: start of digit d within the segment
   if constexpr (SEG)
   {
     slot                   = p.tile_slot[tile];
@@ -561,6 +561,10 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
     const uint64_t seg_end = uint64_t(p.slot_range[2 * slot + 1]);
     tile_base              = seg_begin + uint64_t(tile - first_tile) * kTile;
     left                   = seg_end - tile_base;
+    if (tid < uint32_t(kRadix))
+    {
+      seg_start = p.seg_bins[(uint64_t(slot) * p.num_passes + p.pass) * kRadix + tid];
+    }
   }
   else
   {
@@ -657,9 +661,7 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
   uint64_t gstart      = 0; // SEG, thread d < kRadix: first output position of digit d in the segment
   if constexpr (SEG)
   {
-    const uint32_t* sb = p.seg_bins + (uint64_t(slot) * p.num_passes + p.pass) * kRadix;
-    uint64_t all       = 0;
-    gstart = seg_begin + radix_exclusive_scan<uint64_t, true>(tid < uint32_t(kRadix) ? uint64_t(sb[tid]) : 0, s_scan64, all);
+    gstart = seg_begin + seg_start;
   }
   if (tid < uint32_t(kRadix))
   {
@@ -1207,44 +1209,87 @@ struct SegClassifyParams
 
 // One thread per segment.  List order inside a class follows the atomics and is not
 // deterministic; every segment is sorted independently, so results do not depend on it.
+// Slots are claimed once per wave and class (ballot, popcount, one atomic, lane offset
+// by mbcnt), not once per segment: 100,000 segments on a handful of counters would
+// otherwise serialise on same-address atomics.
 static __global__ void __launch_bounds__(256) seg_classify_kernel(SegClassifyParams p)
 {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= p.num_segments)
+  int64_t b = 0, e = 0;
+  if (i < p.num_segments)
   {
-    return;
-  }
-  int64_t b, e;
-  if (p.offset_bytes == 4)
-  {
-    b = static_cast<const int32_t*>(p.begin_offsets)[i];
-    e = static_cast<const int32_t*>(p.end_offsets)[i];
-  }
-  else
-  {
-    b = static_cast<const int64_t*>(p.begin_offsets)[i];
-    e = static_cast<const int64_t*>(p.end_offsets)[i];
-  }
-  if (e <= b)
-  {
-    return;
-  }
-  const uint64_t size = uint64_t(e - b);
-  for (int c = 0; c < p.num_classes; ++c)
-  {
-    if (size <= p.class_max[c])
+    if (p.offset_bytes == 4)
     {
-      const uint32_t s           = atomicAdd(&p.counters[c], 1u);
-      p.class_range[c][2 * s]     = b;
-      p.class_range[c][2 * s + 1] = e;
-      return;
+      b = static_cast<const int32_t*>(p.begin_offsets)[i];
+      e = static_cast<const int32_t*>(p.end_offsets)[i];
+    }
+    else
+    {
+      b = static_cast<const int64_t*>(p.begin_offsets)[i];
+      e = static_cast<const int64_t*>(p.end_offsets)[i];
     }
   }
-  const uint32_t s         = atomicAdd(&p.counters[kSegLarge], 1u);
-  const uint32_t tiles     = uint32_t((size + p.large_tile - 1) / p.large_tile);
-  p.large_range[2 * s]     = b;
-  p.large_range[2 * s + 1] = e;
-  p.large_first_tile[s]    = atomicAdd(&p.counters[kSegLargeTiles], tiles);
+  const uint64_t size = e > b ? uint64_t(e - b) : 0;
+  // class of this segment: 0..num_classes-1, kSegLarge for the large class, -1 for none
+  int cls = -1;
+  if (size > 0)
+  {
+    cls = kSegLarge;
+    for (int c = p.num_classes - 1; c >= 0; --c)
+    {
+      if (size <= p.class_max[c])
+      {
+        cls = c;
+      }
+    }
+  }
+  const uint32_t lane = lane_id();
+  // one-workgroup classes
+  for (int c = 0; c < p.num_classes; ++c)
+  {
+    const uint64_t m = __ballot(cls == c);
+    if (m == 0)
+    {
+      continue; // wave-uniform
+    }
+    const uint32_t leader = uint32_t(__builtin_ctzll(m));
+    uint32_t base         = 0;
+    if (lane == leader)
+    {
+      base = atomicAdd(&p.counters[c], uint32_t(__builtin_popcountll(m)));
+    }
+    base = __shfl(base, int(leader), 64);
+    if (cls == c)
+    {
+      const uint32_t s            = base + mask_prefix_count(m);
+      p.class_range[c][2 * s]     = b;
+      p.class_range[c][2 * s + 1] = e;
+    }
+  }
+  // large class: slots and a contiguous tile range per segment
+  const uint64_t m = __ballot(cls == kSegLarge);
+  if (m != 0)
+  {
+    const uint32_t leader = uint32_t(__builtin_ctzll(m));
+    const uint32_t tiles  = cls == kSegLarge ? uint32_t((size + p.large_tile - 1) / p.large_tile) : 0u;
+    const uint32_t before = wave_inclusive_scan<uint32_t>(tiles) - tiles; // tiles of lower lanes
+    const uint32_t all    = __shfl(before + tiles, 63, 64);
+    uint32_t slot0 = 0, tile0 = 0;
+    if (lane == leader)
+    {
+      slot0 = atomicAdd(&p.counters[kSegLarge], uint32_t(__builtin_popcountll(m)));
+      tile0 = atomicAdd(&p.counters[kSegLargeTiles], all);
+    }
+    slot0 = __shfl(slot0, int(leader), 64);
+    tile0 = __shfl(tile0, int(leader), 64);
+    if (cls == kSegLarge)
+    {
+      const uint32_t s         = slot0 + mask_prefix_count(m);
+      p.large_range[2 * s]     = b;
+      p.large_range[2 * s + 1] = e;
+      p.large_first_tile[s]    = tile0 + before;
+    }
+  }
 }
 
 struct SegPrepParams
@@ -1284,6 +1329,33 @@ static __global__ void __launch_bounds__(256) seg_prep_kernel(SegPrepParams p)
   for (uint64_t i = threadIdx.x; i < uint64_t(tiles) * kRadix; i += blockDim.x)
   {
     p.lookback[uint64_t(first) * kRadix + i] = 0u;
+  }
+}
+
+// One workgroup per large-segment slot (grid = upper bound): turns the slot's digit
+// counts of every pass into exclusive digit starts (relative to the segment's begin), so
+// each onesweep tile reads them instead of scanning them.
+struct SegStartsParams
+{
+  const uint32_t* counters;
+  uint32_t* seg_bins; // [slot][num_passes][kRadix]
+  int num_passes;
+};
+
+static __global__ void __launch_bounds__(kRadix) seg_starts_kernel(SegStartsParams p)
+{
+  __shared__ uint32_t s_wave[kRadix / 64];
+  const uint32_t slot = blockIdx.x;
+  if (slot >= p.counters[kSegLarge])
+  {
+    return; // block-uniform
+  }
+  for (int pass = 0; pass < p.num_passes; ++pass)
+  {
+    uint32_t* b    = p.seg_bins + (uint64_t(slot) * p.num_passes + pass) * kRadix;
+    uint32_t all   = 0;
+    const uint32_t ex = radix_exclusive_scan<uint32_t>(b[threadIdx.x], s_wave, all);
+    b[threadIdx.x] = ex;
   }
 }
 
