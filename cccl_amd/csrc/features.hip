@@ -1,7 +1,7 @@
 // Input statistics of one sort call for the config selector, computed on the device by
 // one workgroup: segment-size mix from the offset arrays and digit entropies from a
-// strided sample of the keys.  Cost: one small launch and a 64-byte copy, far below
-// the sort itself for the sizes the selector is used on.
+// strided sample of the keys.  Cost: one small launch, far below the sort itself for
+// the sizes the selector is used on.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -12,9 +12,10 @@
 namespace cccl_amd {
 namespace {
 
-constexpr int kFeatThreads = 1024;
+constexpr int kFeatThreads = 256; // == kRadix: thread d owns digit d in the entropy sums
 constexpr int kSample      = 4096;
 constexpr int kMaxPassF    = 8;
+constexpr int kFeatWaves   = kFeatThreads / 64;
 
 struct FeatureParams
 {
@@ -35,10 +36,24 @@ struct FeatureParams
   int pad_;
 };
 
+template <class T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+  for (int o = 32; o > 0; o >>= 1)
+  {
+    v += __shfl_down(v, o, 64);
+  }
+  return v;
+}
+
+// Digit entropies of a strided sample of S = min(n, 4096) keys (key i*n/S), one per
+// digit pass, in bits / 8 (1.0 = uniform 8-bit digits).  All sample loads of a thread
+// are issued before any is used.
 template <class UK>
 __device__ void sample_entropies(const FeatureParams& p, uint32_t* s_hist, float* s_red, float* ent, int passes)
 {
-  const UK* keys = static_cast<const UK*>(p.keys);
+  constexpr int kPer = kSample / kFeatThreads;
+  const UK* keys     = static_cast<const UK*>(p.keys);
   const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
   const uint64_t n = p.num_items;
   const int S      = n < uint64_t(kSample) ? int(n) : kSample;
@@ -46,60 +61,67 @@ __device__ void sample_entropies(const FeatureParams& p, uint32_t* s_hist, float
   {
     s_hist[i] = 0;
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < S; i += kFeatThreads)
+  UK k[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j)
   {
-    const uint64_t idx = (uint64_t(i) * n) / uint64_t(S);
-    const UK t         = tw(keys[idx]);
-    for (int q = 0; q < passes; ++q)
+    const int i = j * kFeatThreads + int(threadIdx.x);
+    k[j]        = i < S ? keys[(uint64_t(i) * n) / uint64_t(S)] : UK(0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kPer; ++j)
+  {
+    if (j * kFeatThreads + int(threadIdx.x) < S)
     {
-      const int shift     = p.begin_bit + q * kRadixBits;
-      const int bits      = min(kRadixBits, p.end_bit - shift);
-      const uint32_t mask = (1u << bits) - 1u;
-      atomicAdd(&s_hist[q * kRadix + extract_digit(t, shift, mask)], 1u);
+      const UK t = tw(k[j]);
+      for (int q = 0; q < passes; ++q)
+      {
+        const int shift     = p.begin_bit + q * kRadixBits;
+        const int bits      = min(kRadixBits, p.end_bit - shift);
+        const uint32_t mask = (1u << bits) - 1u;
+        atomicAdd(&s_hist[q * kRadix + extract_digit(t, shift, mask)], 1u);
+      }
     }
   }
   __syncthreads();
-  // entropy of each pass's digit distribution, in bits / 8 (1.0 = uniform 8-bit digits)
   for (int q = 0; q < passes; ++q)
   {
+    const float c = float(s_hist[q * kRadix + threadIdx.x]);
+    float h       = 0.f;
+    if (c > 0.f)
+    {
+      const float pr = c / float(S);
+      h              = -pr * log2f(pr);
+    }
+    h = wave_sum(h);
+    if ((threadIdx.x & 63u) == 0)
+    {
+      s_red[q * kFeatWaves + (threadIdx.x >> 6)] = h;
+    }
+  }
+  __syncthreads();
+  if (int(threadIdx.x) < passes)
+  {
     float h = 0.f;
-    for (int d = threadIdx.x; d < kRadix; d += kFeatThreads)
+    for (int w = 0; w < kFeatWaves; ++w)
     {
-      const float c = float(s_hist[q * kRadix + d]);
-      if (c > 0.f)
-      {
-        const float pr = c / float(S);
-        h -= pr * log2f(pr);
-      }
+      h += s_red[threadIdx.x * kFeatWaves + w];
     }
-    s_red[threadIdx.x] = h;
-    __syncthreads();
-    for (int w = kFeatThreads / 2; w > 0; w >>= 1)
-    {
-      if (int(threadIdx.x) < w)
-      {
-        s_red[threadIdx.x] += s_red[threadIdx.x + w];
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0)
-    {
-      ent[q] = s_red[0] / 8.f;
-    }
-    __syncthreads();
+    ent[threadIdx.x] = h / 8.f;
   }
 }
 
 __global__ void __launch_bounds__(kFeatThreads) features_kernel(FeatureParams p)
 {
   __shared__ uint32_t s_hist[kMaxPassF * kRadix];
-  __shared__ float s_red[kFeatThreads];
-  __shared__ double s_seg[4][kFeatThreads / 64];
+  __shared__ float s_red[kMaxPassF * kFeatWaves];
+  __shared__ double s_seg[4][kFeatWaves];
   __shared__ float s_ent[kMaxPassF];
 
   // segment-size mix: items in segments of <= 1Ki, (1Ki, 8Ki], (8Ki, 64Ki], > 64Ki
   double acc[4] = {0, 0, 0, 0};
+#pragma unroll 4
   for (int64_t s = threadIdx.x; s < p.num_segments; s += kFeatThreads)
   {
     int64_t b, e;
@@ -118,17 +140,12 @@ __global__ void __launch_bounds__(kFeatThreads) features_kernel(FeatureParams p)
   }
   for (int k = 0; k < 4; ++k)
   {
-    double v = acc[k];
-    for (int o = 32; o > 0; o >>= 1)
-    {
-      v += __shfl_down(v, o, 64);
-    }
+    const double v = wave_sum(acc[k]);
     if ((threadIdx.x & 63u) == 0)
     {
       s_seg[k][threadIdx.x >> 6] = v;
     }
   }
-  __syncthreads();
 
   const int passes = max(1, (p.end_bit - p.begin_bit + kRadixBits - 1) / kRadixBits);
   if (p.num_items > 0 && p.end_bit > p.begin_bit)
@@ -160,7 +177,7 @@ __global__ void __launch_bounds__(kFeatThreads) features_kernel(FeatureParams p)
     double seg[4] = {0, 0, 0, 0}, tot = 0;
     for (int k = 0; k < 4; ++k)
     {
-      for (int w = 0; w < kFeatThreads / 64; ++w)
+      for (int w = 0; w < kFeatWaves; ++w)
       {
         seg[k] += s_seg[k][w];
       }

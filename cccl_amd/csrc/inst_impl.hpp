@@ -29,7 +29,7 @@ KernelEntry onesweep_entry()
   if constexpr (L::kBytes <= kMaxLdsBytes)
   {
     return {
-      int(sizeof(UK)), value_bytes<V>(), T, I, reinterpret_cast<const void*>(&onesweep_kernel<UK, V, T, I, false>), L::kBytes};
+      int(sizeof(UK)), value_bytes<V>(), T, I, reinterpret_cast<const void*>(&onesweep_kernel<UK, V, T, I, kModePlain>), L::kBytes};
   }
   else
   {
@@ -50,12 +50,33 @@ KernelEntry seg_onesweep_entry()
   if constexpr (L::kBytes <= kMaxLdsBytes)
   {
     return {
-      int(sizeof(UK)), value_bytes<V>(), T, I, reinterpret_cast<const void*>(&onesweep_kernel<UK, V, T, I, true>), L::kBytes};
+      int(sizeof(UK)), value_bytes<V>(), T, I, reinterpret_cast<const void*>(&onesweep_kernel<UK, V, T, I, kModeSegLookback>), L::kBytes};
   }
   else
   {
     return {int(sizeof(UK)), value_bytes<V>(), T, I, nullptr, L::kBytes};
   }
+}
+
+template <class UK, class V, int T, int I>
+KernelEntry seg_table_entry()
+{
+  using L = OnesweepLayout<UK, V, T, I>;
+  if constexpr (L::kBytes <= kMaxLdsBytes)
+  {
+    return {int(sizeof(UK)), value_bytes<V>(), T, I,
+            reinterpret_cast<const void*>(&onesweep_kernel<UK, V, T, I, kModeSegTable>), L::kBytes};
+  }
+  else
+  {
+    return {int(sizeof(UK)), value_bytes<V>(), T, I, nullptr, L::kBytes};
+  }
+}
+
+template <class UK, int T, int I>
+KernelEntry seg_upsweep_entry()
+{
+  return {int(sizeof(UK)), 0, T, I, reinterpret_cast<const void*>(&seg_upsweep_kernel<UK, T * I>), 0};
 }
 
 template <class UK, class V, int T, int I>
@@ -118,6 +139,16 @@ KeyWidthTable make_table()
     CCCL_AMD_SEG_ONESWEEP_SHAPES(CCCL_AMD_X)
 #undef CCCL_AMD_X
   };
+  static const KernelEntry seg_table[] = {
+#define CCCL_AMD_X(T, I) CCCL_AMD_ALL_VALUES(seg_table_entry, UK, T, I)
+    CCCL_AMD_SEG_ONESWEEP_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
+  static const KernelEntry seg_upsweep[] = {
+#define CCCL_AMD_X(T, I) seg_upsweep_entry<UK, T, I>(),
+    CCCL_AMD_SEG_ONESWEEP_SHAPES(CCCL_AMD_X)
+#undef CCCL_AMD_X
+  };
   return {upsweep,
           sizeof(upsweep) / sizeof(upsweep[0]),
           onesweep,
@@ -128,7 +159,11 @@ KeyWidthTable make_table()
           seg_onesweep,
           sizeof(seg_onesweep) / sizeof(seg_onesweep[0]),
           seg_pass,
-          sizeof(seg_pass) / sizeof(seg_pass[0])};
+          sizeof(seg_pass) / sizeof(seg_pass[0]),
+          seg_table,
+          sizeof(seg_table) / sizeof(seg_table[0]),
+          seg_upsweep,
+          sizeof(seg_upsweep) / sizeof(seg_upsweep[0])};
 }
 
 } // namespace inst

@@ -37,6 +37,10 @@ struct KeyWidthTable
   size_t num_seg_onesweep;
   const KernelEntry* seg_pass; // one workgroup per segment and pass
   size_t num_seg_pass;
+  const KernelEntry* seg_table; // kModeSegTable downsweep, per look-back shape
+  size_t num_seg_table;
+  const KernelEntry* seg_upsweep; // val_bytes 0, per look-back shape
+  size_t num_seg_upsweep;
 };
 
 const KeyWidthTable& kernels_k1();

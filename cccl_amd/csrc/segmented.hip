@@ -182,10 +182,11 @@ hipError_t segmented_radix_sort(
       d_temp_storage, temp_storage_bytes, keys, vals, pr, num_segments, d_begin_offsets, d_end_offsets, offset_bytes, cfg,
       stream);
   }
-  if (cfg.strategy != 0)
+  if (cfg.strategy != 0 && cfg.strategy != 2)
   {
     return hipErrorInvalidValue;
   }
+  const bool table_passes = cfg.strategy == 2;
 
   const KernelEntry* cls[kMaxBlockClasses] = {};
   uint32_t class_max[kMaxBlockClasses]     = {};
@@ -206,8 +207,11 @@ hipError_t segmented_radix_sort(
     }
   }
   const KernelEntry* large =
-    find_entry(tab->seg_onesweep, tab->num_seg_onesweep, kb, vb, cfg.large_threads, cfg.large_ipt);
-  if (large == nullptr)
+    table_passes ? find_entry(tab->seg_table, tab->num_seg_table, kb, vb, cfg.large_threads, cfg.large_ipt)
+                 : find_entry(tab->seg_onesweep, tab->num_seg_onesweep, kb, vb, cfg.large_threads, cfg.large_ipt);
+  const KernelEntry* large_up =
+    table_passes ? find_entry(tab->seg_upsweep, tab->num_seg_upsweep, kb, 0, cfg.large_threads, cfg.large_ipt) : nullptr;
+  if (large == nullptr || (table_passes && large_up == nullptr))
   {
     set_last_error("segmented: no batched onesweep kernel %dx%d for %d-byte keys, %d-byte values", cfg.large_threads,
                    cfg.large_ipt, kb, vb);
@@ -233,8 +237,8 @@ hipError_t segmented_radix_sort(
     size_t(large_bound) * sizeof(uint32_t), // first tile of each large segment
     size_t(tile_bound) * sizeof(uint32_t), // segment slot of each tile
     size_t(large_bound) * passes * kRadix * sizeof(uint32_t), // digit counts of each large segment
-    size_t(tile_bound) * kRadix * sizeof(uint32_t), // look-back A
-    size_t(tile_bound) * kRadix * sizeof(uint32_t), // look-back B
+    size_t(tile_bound) * kRadix * sizeof(uint32_t), // look-back A, or the tile counts of table passes
+    table_passes ? 0 : size_t(tile_bound) * kRadix * sizeof(uint32_t), // look-back B
     pr.overwrite ? 0 : size_t(n) * (kb + vb), // extra key + value buffer
   };
   for (int c = 0; c < cfg.num_block_classes; ++c)
@@ -323,8 +327,8 @@ hipError_t segmented_radix_sort(
   }
   if (large_bound > 0)
   {
-    SegPrepParams pp{d_counters, d_large_range, d_large_first, d_tile_slot, d_seg_bins, d_lb[0], uint32_t(large_tile),
-                     passes};
+    SegPrepParams pp{d_counters,          d_large_range, d_large_first, d_tile_slot, d_seg_bins,
+                     table_passes ? nullptr : d_lb[0], uint32_t(large_tile), passes};
     if (hipError_t e =
           launch_kernel(reinterpret_cast<const void*>(&seg_prep_kernel), large_bound, 256, 0, stream, &pp, 4);
         e != hipSuccess)
@@ -383,8 +387,69 @@ hipError_t segmented_radix_sort(
     }
   }
 
-  // Large class: one batched onesweep per digit pass.
-  if (large_bound > 0)
+  // Large class, table passes: per digit pass a segmented upsweep, a segmented spine and
+  // a downsweep with XCD-contiguous tiles.
+  if (large_bound > 0 && table_passes)
+  {
+    uint32_t* d_tile_counts = d_lb[0];
+    for (int pass = 0; pass < passes; ++pass)
+    {
+      const int bit       = pr.begin_bit + pass * kRadixBits;
+      const int num_bits  = std::max(0, std::min(pr.end_bit - bit, kRadixBits));
+      const void* k_in    = pass == 0 ? key_in : key_seq[(pass - 1) & 1];
+      SegUpsweepParams up{};
+      up.keys            = k_in;
+      up.total_tiles     = d_counters + kSegLargeTiles;
+      up.tile_slot       = d_tile_slot;
+      up.slot_first_tile = d_large_first;
+      up.slot_range      = d_large_range;
+      up.tile_counts     = d_tile_counts;
+      up.xor_mask        = xor_mask;
+      up.float_mode      = float_mode;
+      up.shift           = bit;
+      up.num_bits        = num_bits;
+      if (hipError_t e = launch_kernel(large_up->fn, tile_bound, kUpsweepThreads, 0, stream, &up, 9); e != hipSuccess)
+      {
+        return e;
+      }
+      SegSpineParams sp{d_counters, d_large_range, d_large_first, uint32_t(large_tile), d_tile_counts};
+      if (hipError_t e =
+            launch_kernel(reinterpret_cast<const void*>(&seg_spine_kernel), large_bound, kRadix, 0, stream, &sp, 10);
+          e != hipSuccess)
+      {
+        return e;
+      }
+      OnesweepParams op{};
+      op.keys_in         = k_in;
+      op.keys_out        = key_seq[pass & 1];
+      op.vals_in         = pass == 0 ? val_in : val_seq[(pass - 1) & 1];
+      op.vals_out        = val_seq[pass & 1];
+      op.num_items       = n;
+      op.xor_mask        = xor_mask;
+      op.num_tiles       = uint32_t(tile_bound);
+      op.float_mode      = float_mode;
+      op.shift           = bit;
+      op.num_bits        = num_bits;
+      op.tile_off        = d_tile_counts;
+      op.num_xcds        = uint32_t(device_xcd_count());
+      op.tile_slot       = d_tile_slot;
+      op.slot_first_tile = d_large_first;
+      op.slot_range      = d_large_range;
+      op.seg_bins        = d_seg_bins;
+      op.total_tiles     = d_counters + kSegLargeTiles;
+      op.pass            = pass;
+      op.num_passes      = passes;
+      op.small_offsets   = n * uint64_t(std::max(kb, vb)) < (uint64_t(1) << 32) ? 1 : 0;
+      if (hipError_t e = launch_kernel(large->fn, tile_bound, large->threads, large->lds_bytes, stream, &op, 6);
+          e != hipSuccess)
+      {
+        return e;
+      }
+    }
+  }
+
+  // Large class, look-back: one batched onesweep per digit pass.
+  if (large_bound > 0 && !table_passes)
   {
     for (int pass = 0; pass < passes; ++pass)
     {
@@ -456,6 +521,7 @@ extern "C" CCCL_C_API void cccl_amd_segmented_default_config(int key_bytes, int 
   {
     cfg->num_block_classes = 3;
   }
-  cfg->large_threads = item > 12 ? 256 : 512;
+  cfg->large_threads = item > 12 ? 256 : (item <= 8 ? 1024 : 512);
   cfg->large_ipt     = 16;
+  cfg->strategy      = 2; // table passes: no look-back chains between a segment's tiles
 }

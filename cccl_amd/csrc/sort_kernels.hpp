@@ -7,9 +7,9 @@
 //   spine_kernel      per-digit exclusive scan of the block counts and the digit starts
 //                     (RadixSortScanBinsKernel, kernel_radix_sort.cuh:147-186)
 //   onesweep_kernel   one digit pass over tiles: wave-ballot rank, LDS-staged scatter.
-//                     SEG == false is the plain sort's downsweep (DeviceRadixSortDownsweepKernel,
-//                     kernel_radix_sort.cuh:244-291) reading its digit starts from the spine;
-//                     SEG == true is the segmented sort's onesweep with decoupled look-back
+//                     kModePlain / kModeSegTable: a downsweep (DeviceRadixSortDownsweepKernel,
+//                     kernel_radix_sort.cuh:244-291) reading its offsets from a spine;
+//                     kModeSegLookback: a onesweep with decoupled look-back
 //                     (DeviceRadixSortOnesweepKernel, kernel_radix_sort.cuh:507-558,
 //                     agent_radix_sort_onesweep.cuh:667-697)
 //   block_sort_kernel whole LSD sort of up to one tile inside one workgroup, one
@@ -353,12 +353,18 @@ static __global__ void __launch_bounds__(kRadix) spine_kernel(SpineParams p)
   }
 }
 
-// One onesweep launch is either one digit pass of a plain sort (SEG == false: tile t =
-// blockIdx.x holds items [t*TILE, (t+1)*TILE) and its digit starts come from the spine)
-// or one digit pass over every large segment of a batched segmented sort (SEG == true:
-// tiles are grouped per segment, the tile range of segment slot s starts at
-// slot_first_tile[s], and a decoupled look-back that stops at the segment's first tile
-// gives the digit starts).
+// Where a digit pass gets each tile's digit offsets from.
+constexpr int kModePlain       = 0; // plain sort: spine tables (digit starts, block and tile prefixes)
+constexpr int kModeSegLookback = 1; // large segments: decoupled look-back within each segment
+constexpr int kModeSegTable    = 2; // large segments: segment digit starts + segmented tile prefixes
+
+// One onesweep launch is one digit pass.  kModePlain: tile t of a plain sort holds items
+// [t*TILE, (t+1)*TILE) and reads its digit offsets from the spine's tables.  The two
+// segmented modes cover every large segment of a batched segmented sort at once: tiles
+// are grouped per segment, the tile range of segment slot s starts at
+// slot_first_tile[s]; kModeSegLookback finds a tile's offsets by a decoupled look-back
+// that stops at the segment's first tile, kModeSegTable reads them from the segmented
+// spine's tile prefixes.
 struct OnesweepParams
 {
   const void* keys_in;
@@ -381,7 +387,7 @@ struct OnesweepParams
   uint32_t num_blocks;
   uint32_t num_xcds; // workgroup w runs on XCD w % num_xcds
 
-  // SEG only
+  // segmented modes (lookback fields: kModeSegLookback only)
   uint32_t* lookback; // [tiles][kRadix], zero on entry
   uint32_t* lookback_next; // cleared here for the next launch (may be null)
   uint32_t* tile_counter; // zero on entry
@@ -472,9 +478,11 @@ __device__ __forceinline__ void store_at(T* base, uint32_t index, const T& v)
   *reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(base) + index * uint32_t(sizeof(T))) = v;
 }
 
-template <class UK, class V, int THREADS, int IPT, bool SEG, bool FLOAT>
+template <class UK, class V, int THREADS, int IPT, int MODE, bool FLOAT>
 __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned char* smem)
 {
+  constexpr bool SEG      = MODE != kModePlain;
+  constexpr bool LOOKBACK = MODE == kModeSegLookback;
   using L              = OnesweepLayout<UK, V, THREADS, IPT>;
   constexpr int kWaves = L::kWaves;
   constexpr int kTile  = L::kTile;
@@ -501,8 +509,8 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
 
   zero_wave_counters(wc);
   uint32_t tile  = blockIdx.x;
-  uint64_t gbase = 0; // plain, thread d < kRadix: first output position of digit d in this tile
-  if constexpr (SEG)
+  uint64_t gbase = 0; // table modes, thread d < kRadix: first output position of digit d in this tile
+  if constexpr (LOOKBACK)
   {
     // Tile ids follow the order in which workgroups start (an atomic counter, not
     // blockIdx), so every tile a look-back waits on belongs to a workgroup that is
@@ -535,15 +543,24 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
     // Workgroups are dispatched to the XCDs round-robin; XCD x takes the x-th contiguous
     // range of tiles, in order.  Consecutive tiles then run on one XCD at about the same
     // time, and their adjacent digit runs meet in that XCD's L2 as whole lines instead
-    // of as partial lines from different L2s.
+    // of as partial lines from different L2s.  (A segmented launch's grid is an upper
+    // bound: workgroups past the real tile count exit.)
+    const uint32_t nt = SEG ? *p.total_tiles : p.num_tiles;
+    if (blockIdx.x >= nt)
+    {
+      return; // block-uniform: spare workgroup
+    }
     const uint32_t nx = p.num_xcds;
     const uint32_t x  = blockIdx.x % nx;
-    tile              = x * (p.num_tiles / nx) + min(x, p.num_tiles % nx) + blockIdx.x / nx;
-    // Issued first: these loads complete while the keys are ranked.
-    if (tid < uint32_t(kRadix))
+    tile              = x * (nt / nx) + min(x, nt % nx) + blockIdx.x / nx;
+    if constexpr (!SEG)
     {
-      gbase = p.digit_start[tid] + p.block_prefix[uint64_t(tid) * p.num_blocks + tile / p.tiles_per_block]
-            + p.tile_off[uint64_t(tile) * kRadix + tid];
+      // Issued first: these loads complete while the keys are ranked.
+      if (tid < uint32_t(kRadix))
+      {
+        gbase = p.digit_start[tid] + p.block_prefix[uint64_t(tid) * p.num_blocks + tile / p.tiles_per_block]
+              + p.tile_off[uint64_t(tile) * kRadix + tid];
+      }
     }
   }
 
@@ -564,6 +581,10 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
     if (tid < uint32_t(kRadix))
     {
       seg_start = p.seg_bins[(uint64_t(slot) * p.num_passes + p.pass) * kRadix + tid];
+      if constexpr (MODE == kModeSegTable)
+      {
+        gbase = seg_begin + seg_start + p.tile_off[uint64_t(tile) * kRadix + tid];
+      }
     }
   }
   else
@@ -648,7 +669,7 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
   }
   const uint32_t pad         = uint32_t(kTile) - valid;
   const uint32_t valid_total = (tid == uint32_t(kRadix - 1)) ? total - pad : total;
-  if constexpr (SEG)
+  if constexpr (LOOKBACK)
   {
     if (tid < uint32_t(kRadix))
     {
@@ -658,8 +679,8 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
   }
   uint32_t tile_items  = 0;
   const uint32_t start = radix_exclusive_scan<uint32_t, true>(total, s_misc + 4, tile_items);
-  uint64_t gstart      = 0; // SEG, thread d < kRadix: first output position of digit d in the segment
-  if constexpr (SEG)
+  uint64_t gstart      = 0; // look-back, thread d < kRadix: first output position of digit d in the segment
+  if constexpr (LOOKBACK)
   {
     gstart = seg_begin + seg_start;
   }
@@ -672,7 +693,7 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
       s_wcnt[w * kRadix + tid] = run;
       run += c[w];
     }
-    if constexpr (!SEG)
+    if constexpr (!LOOKBACK)
     {
       s_base[tid] = gbase - start;
     }
@@ -691,7 +712,7 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
     }
   }
 
-  if constexpr (SEG)
+  if constexpr (LOOKBACK)
   {
     // Decoupled look-back: walk earlier tiles of the segment, adding their counts,
     // until one carries the inclusive prefix.
@@ -761,24 +782,24 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
   }
 }
 
-template <class UK, class V, int THREADS, int IPT, bool SEG>
+template <class UK, class V, int THREADS, int IPT, int MODE>
 __global__ void __launch_bounds__(THREADS) onesweep_kernel(OnesweepParams p)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   if constexpr (sizeof(UK) == 1)
   {
-    onesweep_tile<UK, V, THREADS, IPT, SEG, false>(p, smem); // no 8-bit floating point keys
+    onesweep_tile<UK, V, THREADS, IPT, MODE, false>(p, smem); // no 8-bit floating point keys
   }
   else
   {
     // one block-uniform branch selects the key transform for the whole launch
     if (p.float_mode)
     {
-      onesweep_tile<UK, V, THREADS, IPT, SEG, true>(p, smem);
+      onesweep_tile<UK, V, THREADS, IPT, MODE, true>(p, smem);
     }
     else
     {
-      onesweep_tile<UK, V, THREADS, IPT, SEG, false>(p, smem);
+      onesweep_tile<UK, V, THREADS, IPT, MODE, false>(p, smem);
     }
   }
 }
@@ -1299,7 +1320,7 @@ struct SegPrepParams
   const uint32_t* large_first_tile;
   uint32_t* tile_slot;
   uint32_t* seg_bins; // [slot][passes][kRadix]
-  uint32_t* lookback; // look-back buffer of the first pass: cleared for the slot's tiles
+  uint32_t* lookback; // look-back buffer of the first pass: cleared for the slot's tiles (may be null)
   uint32_t large_tile;
   int num_passes;
 };
@@ -1326,9 +1347,12 @@ static __global__ void __launch_bounds__(256) seg_prep_kernel(SegPrepParams p)
   {
     p.tile_slot[first + t] = slot;
   }
-  for (uint64_t i = threadIdx.x; i < uint64_t(tiles) * kRadix; i += blockDim.x)
+  if (p.lookback != nullptr)
   {
-    p.lookback[uint64_t(first) * kRadix + i] = 0u;
+    for (uint64_t i = threadIdx.x; i < uint64_t(tiles) * kRadix; i += blockDim.x)
+    {
+      p.lookback[uint64_t(first) * kRadix + i] = 0u;
+    }
   }
 }
 
@@ -1356,6 +1380,149 @@ static __global__ void __launch_bounds__(kRadix) seg_starts_kernel(SegStartsPara
     uint32_t all   = 0;
     const uint32_t ex = radix_exclusive_scan<uint32_t>(b[threadIdx.x], s_wave, all);
     b[threadIdx.x] = ex;
+  }
+}
+
+// Large class in kModeSegTable passes: per-tile digit counts of one pass (one workgroup
+// per tile, grid = upper bound), then per digit an exclusive scan over each segment's
+// tiles (the segmented spine), read by the downsweep.
+struct SegUpsweepParams
+{
+  const void* keys;
+  const uint32_t* total_tiles;
+  const uint32_t* tile_slot;
+  const uint32_t* slot_first_tile;
+  const int64_t* slot_range;
+  uint32_t* tile_counts; // [tile][kRadix]
+  uint64_t xor_mask;
+  int float_mode;
+  int shift;
+  int num_bits;
+};
+
+template <class UK, int TILE, bool FLOAT>
+__device__ __forceinline__ void seg_upsweep_tile(const SegUpsweepParams& p, uint32_t* s_cnt)
+{
+  constexpr int kThreads  = kUpsweepThreads;
+  constexpr int kPerThread = TILE / kThreads;
+  constexpr int kBatch    = kPerThread < 16 ? kPerThread : 16;
+  static_assert(TILE % kThreads == 0 && kPerThread % kBatch == 0, "tile must split evenly over the threads");
+  const uint32_t tid        = threadIdx.x;
+  const uint32_t tile       = blockIdx.x;
+  const uint32_t slot       = p.tile_slot[tile];
+  const uint64_t seg_end    = uint64_t(p.slot_range[2 * slot + 1]);
+  const uint64_t base       = uint64_t(p.slot_range[2 * slot]) + uint64_t(tile - p.slot_first_tile[slot]) * TILE;
+  const uint32_t valid      = seg_end - base < uint64_t(TILE) ? uint32_t(seg_end - base) : uint32_t(TILE);
+  const UK* keys            = static_cast<const UK*>(p.keys) + base;
+  const TwiddleT<UK, FLOAT> tw{UK(p.xor_mask)};
+  const uint32_t dmask      = (1u << p.num_bits) - 1u;
+  s_cnt[tid]                = 0;
+  lds_barrier();
+  for (int j0 = 0; j0 < kPerThread; j0 += kBatch)
+  {
+    UK k[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j)
+    {
+      const uint32_t i = uint32_t(j0 + j) * kThreads + tid;
+      k[j]             = i < valid ? keys[i] : UK(0);
+    }
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j)
+    {
+      if (uint32_t(j0 + j) * kThreads + tid < valid)
+      {
+        const uint32_t d  = extract_digit(tw(k[j]), p.shift, dmask);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        const uint64_t eq = __ballot(d == d0);
+        if (eq == __builtin_amdgcn_read_exec()) // wave-uniform
+        {
+          if (mask_prefix_count(eq) == 0u)
+          {
+            __hip_atomic_fetch_add(s_cnt + d0, uint32_t(__builtin_popcountll(eq)), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        }
+        else
+        {
+          __hip_atomic_fetch_add(s_cnt + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+    }
+  }
+  lds_barrier();
+  p.tile_counts[uint64_t(tile) * kRadix + tid] = s_cnt[tid];
+}
+
+template <class UK, int TILE>
+__global__ void __launch_bounds__(kUpsweepThreads) seg_upsweep_kernel(SegUpsweepParams p)
+{
+  __shared__ uint32_t s_cnt[kRadix];
+  if (blockIdx.x >= *p.total_tiles)
+  {
+    return; // block-uniform: spare workgroup
+  }
+  if constexpr (sizeof(UK) == 1)
+  {
+    seg_upsweep_tile<UK, TILE, false>(p, s_cnt);
+  }
+  else
+  {
+    if (p.float_mode)
+    {
+      seg_upsweep_tile<UK, TILE, true>(p, s_cnt);
+    }
+    else
+    {
+      seg_upsweep_tile<UK, TILE, false>(p, s_cnt);
+    }
+  }
+}
+
+struct SegSpineParams
+{
+  const uint32_t* counters;
+  const int64_t* slot_range;
+  const uint32_t* slot_first_tile;
+  uint32_t large_tile;
+  uint32_t* tile_counts; // [tile][kRadix], replaced in place by the exclusive prefixes
+};
+
+// One workgroup per large segment (grid = upper bound on their number), thread d owns
+// digit d: an exclusive scan of the segment's tile counts, read one 1 KiB row per tile
+// (coalesced), kSpineBatch rows in flight at a time.  Segments are independent, so the
+// segmented scan needs no carries between workgroups.
+constexpr int kSpineBatch = 16;
+
+static __global__ void __launch_bounds__(kRadix) seg_spine_kernel(SegSpineParams p)
+{
+  const uint32_t slot = blockIdx.x;
+  if (slot >= p.counters[kSegLarge])
+  {
+    return; // block-uniform: spare workgroup
+  }
+  const uint64_t size  = uint64_t(p.slot_range[2 * slot + 1] - p.slot_range[2 * slot]);
+  const uint32_t t0    = p.slot_first_tile[slot];
+  const uint32_t t1    = t0 + uint32_t((size + p.large_tile - 1) / p.large_tile);
+  const uint32_t d     = threadIdx.x;
+  uint32_t run         = 0;
+  for (uint32_t b = t0; b < t1; b += kSpineBatch)
+  {
+    uint32_t c[kSpineBatch];
+#pragma unroll
+    for (int j = 0; j < kSpineBatch; ++j)
+    {
+      c[j] = b + uint32_t(j) < t1 ? p.tile_counts[uint64_t(b + j) * kRadix + d] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kSpineBatch; ++j)
+    {
+      if (b + uint32_t(j) < t1)
+      {
+        p.tile_counts[uint64_t(b + j) * kRadix + d] = run;
+        run += c[j];
+      }
+    }
   }
 }
 

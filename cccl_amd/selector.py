@@ -79,6 +79,8 @@ def _segmented_arms() -> list[Arm]:
     for cname, classes in class_sets.items():
         for large in LOOKBACK_SHAPES:
             arms.append(Arm(f"seg {cname} L{large[0]}x{large[1]}", "segmented", block_classes=classes, large=large))
+            arms.append(Arm(f"seg {cname} T{large[0]}x{large[1]}", "segmented", block_classes=classes, large=large,
+                            strategy=2))
     for large in [(256, 16), (512, 8)]:
         arms.append(Arm(f"seg per-segment-pass {large[0]}x{large[1]}", "segmented", large=large, strategy=1))
     return arms

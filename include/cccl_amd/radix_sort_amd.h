@@ -48,9 +48,12 @@ typedef struct cccl_amd_radix_sort_config_t
  * block_threads[c]*block_ipt[c] items, each sorted whole by one workgroup; larger
  * segments are sorted together by one batched onesweep launch per digit pass with
  * tiles of large_threads*large_ipt items.
- * strategy 1 instead runs CUB's DeviceSegmentedRadixSort schedule: one launch per digit
- * pass, one workgroup per segment looping over its tiles of large_threads*large_ipt
- * items (the block classes are ignored). */
+ * strategy 0 finds the large segments' tile offsets by a decoupled look-back inside
+ * each segment (CUB's onesweep); strategy 2 instead runs per digit pass a per-tile
+ * upsweep, a per-segment scan of the tile counts and a downsweep.  strategy 1 runs CUB's
+ * DeviceSegmentedRadixSort schedule: one launch per digit pass, one workgroup per
+ * segment looping over its tiles of large_threads*large_ipt items (the block classes
+ * are ignored). */
 typedef struct cccl_amd_segmented_config_t
 {
   int32_t num_block_classes; /* 0..4 */
@@ -58,7 +61,8 @@ typedef struct cccl_amd_segmented_config_t
   int32_t block_ipt[4];
   int32_t large_threads;
   int32_t large_ipt;
-  int32_t strategy; /* 0 size-binned batch, 1 one workgroup per segment per pass */
+  int32_t strategy; /* 0 size-binned batch with look-back, 2 size-binned batch with table passes,
+                     * 1 one workgroup per segment per pass */
   int32_t reserved;
 } cccl_amd_segmented_config_t;
 
@@ -154,8 +158,9 @@ CCCL_C_API int cccl_amd_graph_destroy(void* graph_exec);
 /* Kernel timing for benchmarks: while enabled, every kernel the sorts launch is
  * bracketed by a pair of HIP events on its own stream.  read waits for the recorded
  * events, writes up to max (kind, milliseconds) pairs and returns how many there were.
- * Kinds: 0 init, 1 histogram, 2 onesweep pass, 3 one-workgroup sort, 4 segmented
- * binning, 5 segmented histogram, 6 segmented onesweep pass, 7 copy, 8 digit-start scan. */
+ * Kinds: 0 init, 1 upsweep, 2 digit pass (downsweep or onesweep), 3 one-workgroup sort,
+ * 4 segmented binning, 5 segmented histogram and digit starts, 6 segmented digit pass,
+ * 7 copy, 8 spine (bin scan), 9 segmented upsweep, 10 segmented spine. */
 CCCL_C_API void cccl_amd_profile_enable(int on);
 CCCL_C_API int cccl_amd_profile_read(int* kinds, float* ms, int max);
 

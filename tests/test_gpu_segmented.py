@@ -181,24 +181,40 @@ def test_per_segment_pass_schedule(large):
     check(keys, None, sb, se, begin_bit=3, end_bit=21, config=_cfg([], large, strategy=1))
 
 
+@pytest.mark.parametrize("strategy", [0, 2])
 @pytest.mark.parametrize("classes,large", [
-    ([], (512, 16)),                                   # everything through the batched onesweep
+    ([], (512, 16)),                                   # everything through the large class
     ([(256, 4)], (256, 16)),
     ([(256, 8), (512, 16)], (512, 8)),
     ([(256, 4), (256, 8), (256, 16), (512, 16)], (512, 16)),
+    ([(256, 8), (512, 16)], (1024, 16)),
 ])
-def test_class_routings(classes, large):
-    rng = np.random.default_rng(len(classes))
+def test_class_routings(classes, large, strategy):
+    """Look-back (strategy 0) and table passes (strategy 2) for the large class."""
+    rng = np.random.default_rng(len(classes) + large[0])
     sizes = lognormal_segments(rng, 1500, 512, 2.0, 1, 300000)
     sb, se = contiguous_offsets(sizes)
     n = int(se[-1])
     keys = random_keys(rng, np.uint32, n, "and3")
     vals = np.arange(n, dtype=np.uint32)
-    check(keys, vals, sb, se, config=_cfg(classes, large))
-    check(keys, vals, sb, se, descending=True, overwrite=True, config=_cfg(classes, large))
+    check(keys, vals, sb, se, config=_cfg(classes, large, strategy))
+    check(keys, vals, sb, se, descending=True, overwrite=True, config=_cfg(classes, large, strategy))
 
 
-def test_two_large_segments():
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32, np.int64])
+def test_table_passes_key_types(dtype):
+    rng = np.random.default_rng(91)
+    sizes = lognormal_segments(rng, 600, 2048, 2.0, 0, 200000)
+    sb, se = contiguous_offsets(sizes)
+    n = int(se[-1])
+    keys = random_keys(rng, dtype, n)
+    vals = np.arange(n, dtype=np.uint32)
+    check(keys, vals, sb, se, config=_cfg([(256, 16)], (256, 16), 2))
+    check(keys, None, sb, se, descending=True, config=_cfg([], (512, 8), 2))
+
+
+@pytest.mark.parametrize("strategy", [0, 2])
+def test_two_large_segments(strategy):
     """catch2_test_device_segmented_radix_sort_keys.cu: two very large segments."""
     rng = np.random.default_rng(77)
     n = 3_000_000
@@ -206,7 +222,8 @@ def test_two_large_segments():
     vals = np.arange(n, dtype=np.uint32)
     sb = np.array([0, 1_700_001])
     se = np.array([1_700_001, n])
-    check(keys, vals, sb, se)
+    cfg = None if strategy == 0 else _cfg([(256, 16)], (512, 16), 2)
+    check(keys, vals, sb, se, config=cfg)
 
 
 def test_many_tiny_segments():
