@@ -288,6 +288,35 @@ __device__ __forceinline__ void wave_rank_packed(uint32_t (&rd)[IPT], uint32_t* 
   }
 }
 
+// Adds one to counters[idx] for every active lane.  When more than kLdsCountShared lanes
+// share the first active lane's counter -- all of them for constant keys, most of them
+// for skewed digit distributions -- one of them adds their count with one atomic
+// (same-address LDS atomics of a wave are serialised) and the other lanes add one each;
+// otherwise (spread keys) every lane adds one, a single atomic instruction per wave.
+constexpr uint32_t kLdsCountShared = 4;
+
+__device__ __forceinline__ void lds_count(uint32_t* counters, uint32_t idx)
+{
+  const uint32_t idx0 = __builtin_amdgcn_readfirstlane(idx);
+  const uint64_t eq   = __ballot(idx == idx0);
+  if (uint32_t(__builtin_popcountll(eq)) <= kLdsCountShared)
+  {
+    __hip_atomic_fetch_add(counters + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  else if (idx == idx0)
+  {
+    if (mask_prefix_count(eq) == 0u)
+    {
+      __hip_atomic_fetch_add(counters + idx0, uint32_t(__builtin_popcountll(eq)), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  else
+  {
+    __hip_atomic_fetch_add(counters + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
+
 // Zeroes one wave's kRadix counters; only that wave touches them until the next
 // workgroup barrier, so no barrier is needed before its rank.
 __device__ __forceinline__ void zero_wave_counters(uint32_t* wc)

@@ -108,8 +108,8 @@ constexpr int kUpsweepThreads        = 256;
 // keys are streamed as one range (tile boundaries need no barrier): every key adds one
 // to the LDS counter of (its tile, its digit).  A 16-byte aligned array is read with
 // 16-byte loads, four per thread per batch, the next batch in flight while the current
-// one is counted; a wave's 64 loads are 1 KiB of one tile.  When all 64 lanes hit the
-// same counter (low-entropy keys) one lane adds 64 instead of 64 conflicting atomics.
+// one is counted; a wave's 64 loads are 1 KiB of one tile.  lds_count adds the lanes
+// that share the first lane's counter (low-entropy keys) with one atomic.
 // At the end each digit's counts are scanned over the block's tiles into tile_off rows
 // and the block sum.
 template <class UK, int TILE, bool FLOAT>
@@ -135,21 +135,7 @@ __device__ __forceinline__ void upsweep_block(const UpsweepParams& p, uint32_t* 
 
   // idx = tile within the block * kRadix + digit
   auto add = [&](UK k, uint32_t tile_local) {
-    const uint32_t idx  = tile_local * kRadix + extract_digit(tw(k), p.shift, dmask);
-    const uint32_t idx0 = __builtin_amdgcn_readfirstlane(idx);
-    const uint64_t eq   = __ballot(idx == idx0);
-    if (eq == __builtin_amdgcn_read_exec()) // wave-uniform
-    {
-      if (mask_prefix_count(eq) == 0u)
-      {
-        __hip_atomic_fetch_add(s_cnt + idx0, uint32_t(__builtin_popcountll(eq)), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-    }
-    else
-    {
-      __hip_atomic_fetch_add(s_cnt + idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
+    lds_count(s_cnt, tile_local * kRadix + extract_digit(tw(k), p.shift, dmask));
   };
 
   uint64_t scalar_from = begin; // keys before this position have been counted
@@ -1432,21 +1418,7 @@ __device__ __forceinline__ void seg_upsweep_tile(const SegUpsweepParams& p, uint
     {
       if (uint32_t(j0 + j) * kThreads + tid < valid)
       {
-        const uint32_t d  = extract_digit(tw(k[j]), p.shift, dmask);
-        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-        const uint64_t eq = __ballot(d == d0);
-        if (eq == __builtin_amdgcn_read_exec()) // wave-uniform
-        {
-          if (mask_prefix_count(eq) == 0u)
-          {
-            __hip_atomic_fetch_add(s_cnt + d0, uint32_t(__builtin_popcountll(eq)), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-          }
-        }
-        else
-        {
-          __hip_atomic_fetch_add(s_cnt + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
+        lds_count(s_cnt, extract_digit(tw(k[j]), p.shift, dmask));
       }
     }
   }
