@@ -307,17 +307,31 @@ cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64
 {
   cccl_amd_radix_sort_config_t c{};
   c.radix_bits            = kRadixBits;
-  c.upsweep_blocks_per_cu = 8;
+  c.upsweep_blocks_per_cu = 4;
   c.single_tile_threads   = 256;
   c.single_tile_ipt       = 16;
   const int item_bytes    = key_bytes + val_bytes;
   if (num_items >= (uint64_t(1) << 22))
   {
-    // Pairs of up to 8 bytes: the largest tile (longest digit runs per tile, so the
-    // scatter writes fewer partial lines); keys only: 512 x 16 (two workgroups per CU
-    // hide the rank's latency better).
-    c.onesweep_threads = item_bytes > 12 ? 256 : (val_bytes > 0 && item_bytes <= 8 ? 1024 : 512);
-    c.onesweep_ipt     = 16;
+    // Timed at 2^27-2^28 items.  32-bit keys with 32-bit values: the largest tile
+    // (longest digit runs per tile, so the scatter writes fewer partial lines); pairs of
+    // 12 to 16 bytes: 1024 x 8; keys only and narrower pairs: 512 x 16 (two workgroups
+    // per CU hide the rank's latency better); wider pairs: 256 x 16.
+    if (key_bytes == 4 && val_bytes == 4)
+    {
+      c.onesweep_threads = 1024;
+      c.onesweep_ipt     = 16;
+    }
+    else if (val_bytes > 0 && item_bytes >= 12)
+    {
+      c.onesweep_threads = item_bytes > 16 ? 256 : 1024;
+      c.onesweep_ipt     = item_bytes > 16 ? 16 : 8;
+    }
+    else
+    {
+      c.onesweep_threads = 512;
+      c.onesweep_ipt     = 16;
+    }
   }
   else
   {

@@ -12,10 +12,12 @@
 namespace cccl_amd {
 namespace {
 
-constexpr int kFeatThreads = 256; // == kRadix: thread d owns digit d in the entropy sums
+constexpr int kFeatThreads = 1024;
 constexpr int kSample      = 4096;
 constexpr int kMaxPassF    = 8;
 constexpr int kFeatWaves   = kFeatThreads / 64;
+constexpr int kDigitWaves  = kRadix / 64; // threads d < kRadix own digit d in the entropy sums
+constexpr int kSegBatch    = 8;           // offset pairs a thread has in flight at once
 
 struct FeatureParams
 {
@@ -57,10 +59,6 @@ __device__ void sample_entropies(const FeatureParams& p, uint32_t* s_hist, float
   const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
   const uint64_t n = p.num_items;
   const int S      = n < uint64_t(kSample) ? int(n) : kSample;
-  for (int i = threadIdx.x; i < kMaxPassF * kRadix; i += kFeatThreads)
-  {
-    s_hist[i] = 0;
-  }
   UK k[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j)
@@ -68,7 +66,6 @@ __device__ void sample_entropies(const FeatureParams& p, uint32_t* s_hist, float
     const int i = j * kFeatThreads + int(threadIdx.x);
     k[j]        = i < S ? keys[(uint64_t(i) * n) / uint64_t(S)] : UK(0);
   }
-  __syncthreads();
 #pragma unroll
   for (int j = 0; j < kPer; ++j)
   {
@@ -85,68 +82,95 @@ __device__ void sample_entropies(const FeatureParams& p, uint32_t* s_hist, float
     }
   }
   __syncthreads();
-  for (int q = 0; q < passes; ++q)
+  if (threadIdx.x < uint32_t(kRadix)) // whole waves
   {
-    const float c = float(s_hist[q * kRadix + threadIdx.x]);
-    float h       = 0.f;
-    if (c > 0.f)
+    for (int q = 0; q < passes; ++q)
     {
-      const float pr = c / float(S);
-      h              = -pr * log2f(pr);
-    }
-    h = wave_sum(h);
-    if ((threadIdx.x & 63u) == 0)
-    {
-      s_red[q * kFeatWaves + (threadIdx.x >> 6)] = h;
+      const float c = float(s_hist[q * kRadix + threadIdx.x]);
+      float h       = 0.f;
+      if (c > 0.f)
+      {
+        const float pr = c / float(S);
+        h              = -pr * log2f(pr);
+      }
+      h = wave_sum(h);
+      if ((threadIdx.x & 63u) == 0)
+      {
+        s_red[q * kDigitWaves + (threadIdx.x >> 6)] = h;
+      }
     }
   }
   __syncthreads();
   if (int(threadIdx.x) < passes)
   {
     float h = 0.f;
-    for (int w = 0; w < kFeatWaves; ++w)
+    for (int w = 0; w < kDigitWaves; ++w)
     {
-      h += s_red[threadIdx.x * kFeatWaves + w];
+      h += s_red[threadIdx.x * kDigitWaves + w];
     }
     ent[threadIdx.x] = h / 8.f;
+  }
+}
+
+// Item counts per segment-size bucket (<= 1Ki, (1Ki, 8Ki], (8Ki, 64Ki], > 64Ki), over
+// segments tid, tid + T, ...; kSegBatch begin/end pairs are loaded before any is used.
+template <class OffsetT>
+__device__ void segment_mix(const FeatureParams& p, uint64_t (&acc)[4])
+{
+  const OffsetT* bo  = static_cast<const OffsetT*>(p.begin_offsets);
+  const OffsetT* eo  = static_cast<const OffsetT*>(p.end_offsets);
+  const int64_t step = int64_t(kFeatThreads) * kSegBatch;
+  for (int64_t s0 = threadIdx.x; s0 < p.num_segments; s0 += step)
+  {
+    OffsetT b[kSegBatch], e[kSegBatch];
+#pragma unroll
+    for (int j = 0; j < kSegBatch; ++j)
+    {
+      const int64_t s = s0 + int64_t(j) * kFeatThreads;
+      b[j]            = s < p.num_segments ? bo[s] : OffsetT(0);
+      e[j]            = s < p.num_segments ? eo[s] : OffsetT(0);
+    }
+#pragma unroll
+    for (int j = 0; j < kSegBatch; ++j)
+    {
+      const uint64_t sz = e[j] > b[j] ? uint64_t(int64_t(e[j]) - int64_t(b[j])) : 0u;
+      acc[sz <= 1024u ? 0 : sz <= 8192u ? 1 : sz <= 65536u ? 2 : 3] += sz;
+    }
   }
 }
 
 __global__ void __launch_bounds__(kFeatThreads) features_kernel(FeatureParams p)
 {
   __shared__ uint32_t s_hist[kMaxPassF * kRadix];
-  __shared__ float s_red[kMaxPassF * kFeatWaves];
-  __shared__ double s_seg[4][kFeatWaves];
+  __shared__ float s_red[kMaxPassF * kDigitWaves];
+  __shared__ uint64_t s_seg[4][kFeatWaves];
   __shared__ float s_ent[kMaxPassF];
 
-  // segment-size mix: items in segments of <= 1Ki, (1Ki, 8Ki], (8Ki, 64Ki], > 64Ki
-  double acc[4] = {0, 0, 0, 0};
-#pragma unroll 4
-  for (int64_t s = threadIdx.x; s < p.num_segments; s += kFeatThreads)
+  for (int i = threadIdx.x; i < kMaxPassF * kRadix; i += kFeatThreads)
   {
-    int64_t b, e;
+    s_hist[i] = 0;
+  }
+  uint64_t acc[4] = {0, 0, 0, 0};
+  if (p.num_segments > 0)
+  {
     if (p.offset_bytes == 4)
     {
-      b = static_cast<const int32_t*>(p.begin_offsets)[s];
-      e = static_cast<const int32_t*>(p.end_offsets)[s];
+      segment_mix<int32_t>(p, acc);
     }
     else
     {
-      b = static_cast<const int64_t*>(p.begin_offsets)[s];
-      e = static_cast<const int64_t*>(p.end_offsets)[s];
+      segment_mix<int64_t>(p, acc);
     }
-    const double sz = e > b ? double(e - b) : 0.0;
-    acc[sz <= 1024.0 ? 0 : sz <= 8192.0 ? 1 : sz <= 65536.0 ? 2 : 3] += sz;
   }
   for (int k = 0; k < 4; ++k)
   {
-    const double v = wave_sum(acc[k]);
+    const uint64_t v = wave_sum(acc[k]);
     if ((threadIdx.x & 63u) == 0)
     {
       s_seg[k][threadIdx.x >> 6] = v;
     }
   }
-
+  __syncthreads(); // counters cleared
   const int passes = max(1, (p.end_bit - p.begin_bit + kRadixBits - 1) / kRadixBits);
   if (p.num_items > 0 && p.end_bit > p.begin_bit)
   {
@@ -171,10 +195,9 @@ __global__ void __launch_bounds__(kFeatThreads) features_kernel(FeatureParams p)
     s_ent[threadIdx.x] = 0.f;
   }
   __syncthreads();
-
   if (threadIdx.x == 0)
   {
-    double seg[4] = {0, 0, 0, 0}, tot = 0;
+    uint64_t seg[4] = {0, 0, 0, 0}, tot = 0;
     for (int k = 0; k < 4; ++k)
     {
       for (int w = 0; w < kFeatWaves; ++w)
@@ -203,7 +226,7 @@ __global__ void __launch_bounds__(kFeatThreads) features_kernel(FeatureParams p)
     o[7]     = segmented ? log2f(float(p.num_segments) + 1.f) / 20.f : 0.f;
     for (int k = 0; k < 4; ++k)
     {
-      o[8 + k] = segmented && tot > 0 ? float(seg[k] / tot) : (k == 3 ? 1.f : 0.f);
+      o[8 + k] = segmented && tot > 0 ? float(double(seg[k]) / double(tot)) : (k == 3 ? 1.f : 0.f);
     }
     o[12] = mean;
     o[13] = mn;

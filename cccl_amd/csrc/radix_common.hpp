@@ -266,6 +266,7 @@ __device__ __forceinline__ void wave_rank(const uint32_t (&dig)[IPT], uint32_t (
     const uint32_t cnt   = uint32_t(__builtin_popcountll(peers));
     const uint32_t old   = lds_load_wave(wc + dig[i]);
     rank[i]              = old + below;
+    asm volatile("" : "+v"(rank[i])); // computed here, not at its use (see wave_rank_packed)
     lds_store_wave(wc + dig[i], old + cnt);
   }
 }
@@ -284,6 +285,10 @@ __device__ __forceinline__ void wave_rank_packed(uint32_t (&rd)[IPT], uint32_t* 
     const uint32_t cnt   = uint32_t(__builtin_popcountll(peers));
     const uint32_t old   = lds_load_wave(wc + d);
     rd[i]                = (old + below) | (d << 16);
+    // Materialise the packed word here: otherwise the compiler defers it to its use
+    // and keeps the peer mask, the old count and the digit of every item live (four
+    // registers per item instead of one), which spills at large IPT.
+    asm volatile("" : "+v"(rd[i]));
     lds_store_wave(wc + d, old + cnt);
   }
 }

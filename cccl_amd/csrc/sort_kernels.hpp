@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "radix_common.hpp"
+#include "registry.hpp"
 
 namespace cccl_amd {
 
@@ -107,7 +108,7 @@ constexpr int kUpsweepThreads        = 256;
 // One workgroup per block of at most kMaxTilesPerBlock consecutive tiles.  The block's
 // keys are streamed as one range (tile boundaries need no barrier): every key adds one
 // to the LDS counter of (its tile, its digit).  A 16-byte aligned array is read with
-// 16-byte loads, four per thread per batch, the next batch in flight while the current
+// 16-byte loads, two per thread per batch, the next batch in flight while the current
 // one is counted; a wave's 64 loads are 1 KiB of one tile.  lds_count adds the lanes
 // that share the first lane's counter (low-entropy keys) with one atomic.
 // At the end each digit's counts are scanned over the block's tiles into tile_off rows
@@ -145,7 +146,7 @@ __device__ __forceinline__ void upsweep_block(const UpsweepParams& p, uint32_t* 
     {
       constexpr int kPer               = int(16 / sizeof(UK));
       constexpr uint32_t kVecPerTile   = uint32_t(TILE / kPer);
-      constexpr int kU                 = 4;
+      constexpr int kU                 = 2;
       constexpr uint32_t kStep         = uint32_t(kU * kThreads);
       const uint4* src                 = reinterpret_cast<const uint4*>(keys + begin);
       const uint32_t nvec              = uint32_t((end - begin) / kPer);
@@ -399,8 +400,15 @@ struct OnesweepLayout
   static constexpr size_t kOffBase  = kWcnt;
   static constexpr size_t kOffMisc  = kOffBase + kBase;
   static constexpr size_t kOffKeys  = kOffMisc + kMisc;
-  static constexpr size_t kOffVals  = kOffKeys + size_t(kTile) * sizeof(UK);
-  static constexpr size_t kBytes    = kOffVals + (kHasV ? size_t(kTile) * sizeof(V) : 0);
+  static constexpr size_t kKeyBytes = size_t(kTile) * sizeof(UK);
+  static constexpr size_t kValBytes = kHasV ? size_t(kTile) * sizeof(V) : 0;
+  // Keys and values are staged side by side when both fit in one workgroup's LDS;
+  // otherwise (large tiles) the values are staged in the keys' space once the keys have
+  // been written out.
+  static constexpr bool kTwoRound  = kHasV && kOffKeys + kKeyBytes + kValBytes > kMaxLdsBytes;
+  static constexpr size_t kOffVals = kTwoRound ? kOffKeys : kOffKeys + kKeyBytes;
+  static constexpr size_t kBytes   = kTwoRound ? kOffKeys + (kKeyBytes > kValBytes ? kKeyBytes : kValBytes)
+                                               : kOffVals + kValBytes;
 };
 
 // Sum of digit d's counts over tiles [first_tile, tile) from the look-back status words.
@@ -686,13 +694,19 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
   }
   lds_barrier();
 
-  // Stage keys and values in LDS in digit order.
+  // Stage keys and values in LDS in digit order.  With two rounds only the keys go now;
+  // rd[i] keeps each item's LDS position for its value.
+  constexpr bool kTwoRound = L::kTwoRound;
 #pragma unroll
   for (int i = 0; i < IPT; ++i)
   {
     const uint32_t pos = wc[rd[i] >> 16] + (rd[i] & 0xffffu);
     s_keys[pos]        = keys[i];
-    if constexpr (kHasV)
+    if constexpr (kTwoRound)
+    {
+      rd[i] = pos;
+    }
+    else if constexpr (kHasV)
     {
       s_vals[pos] = vals[i];
     }
@@ -715,8 +729,11 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
   lds_barrier();
 
   // Write out in LDS order: consecutive threads take consecutive staged items, which
-  // land at consecutive addresses within each digit's run.
-  if (full && p.small_offsets)
+  // land at consecutive addresses within each digit's run.  With two rounds the digit of
+  // each staged item is kept (four per word) for its value.
+  uint32_t dig[kTwoRound ? (IPT + 3) / 4 : 1] = {};
+  const bool fast = full && p.small_offsets;
+  if (fast)
   {
     // Full tile below 4 GiB: no bounds checks and 32-bit offsets (the low word of the
     // 64-bit base is exact modulo 2^32, and base + pos fits in 32 bits).
@@ -735,10 +752,15 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
       for (int j = 0; j < kChunk; ++j)
       {
         const uint32_t pos = uint32_t(c0 + j) * THREADS + tid;
-        at[j]              = s_base32[2 * extract_digit(tw(k[j]), p.shift, dmask)] + pos;
+        const uint32_t d   = extract_digit(tw(k[j]), p.shift, dmask);
+        at[j]              = s_base32[2 * d] + pos;
         store_at(kout, at[j], k[j]);
+        if constexpr (kTwoRound)
+        {
+          dig[(c0 + j) / 4] |= d << (8 * ((c0 + j) % 4));
+        }
       }
-      if constexpr (kHasV)
+      if constexpr (kHasV && !kTwoRound)
       {
 #pragma unroll
         for (int j = 0; j < kChunk; ++j)
@@ -757,12 +779,43 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
       if (pos < valid)
       {
         const UK k        = s_keys[pos];
-        const uint64_t at = s_base[extract_digit(tw(k), p.shift, dmask)] + pos;
+        const uint32_t d  = extract_digit(tw(k), p.shift, dmask);
+        const uint64_t at = s_base[d] + pos;
         kout[at]          = k;
-        if constexpr (kHasV)
+        if constexpr (kTwoRound)
+        {
+          dig[i / 4] |= d << (8 * (i % 4));
+        }
+        else if constexpr (kHasV)
         {
           vout[at] = s_vals[pos];
         }
+      }
+    }
+  }
+  if constexpr (kTwoRound)
+  {
+    // Second round: every key has been read out of the shared space; stage the values
+    // there at the keys' positions and write them out the same way.
+    lds_barrier();
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      s_vals[rd[i]] = vals[i];
+    }
+    lds_barrier();
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      const uint32_t pos = uint32_t(i) * THREADS + tid;
+      const uint32_t d   = (dig[i / 4] >> (8 * (i % 4))) & 0xffu;
+      if (fast)
+      {
+        store_at(vout, s_base32[2 * d] + pos, s_vals[pos]);
+      }
+      else if (pos < valid)
+      {
+        vout[s_base[d] + pos] = s_vals[pos];
       }
     }
   }

@@ -102,7 +102,9 @@ def _item_bytes(t) -> int:
 
 def _stream_handle(stream) -> int | None:
     if stream is None:
-        return torch.cuda.current_stream().cuda_stream
+        # torch's current stream of the current device as a raw handle; the raw query
+        # skips building a Stream object (microseconds per call on small sorts)
+        return torch._C._cuda_getCurrentRawStream(torch.cuda.current_device())
     if isinstance(stream, int):
         return stream
     return stream.cuda_stream
@@ -213,7 +215,7 @@ ALGO_ONESWEEP = 1  # upfront histogram + one decoupled-look-back kernel per digi
 
 
 def make_config(onesweep_threads, onesweep_ipt, single_tile_threads=256, single_tile_ipt=16,
-                upsweep_blocks_per_cu=8, algorithm=ALGO_PASSES) -> B.radix_sort_config:
+                upsweep_blocks_per_cu=4, algorithm=ALGO_PASSES) -> B.radix_sort_config:
     c = B.radix_sort_config()
     c.onesweep_threads, c.onesweep_ipt, c.radix_bits = onesweep_threads, onesweep_ipt, 8
     c.single_tile_threads, c.single_tile_ipt = single_tile_threads, single_tile_ipt

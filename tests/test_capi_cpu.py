@@ -208,8 +208,10 @@ def test_configs_and_shapes():
     assert not ca.config_valid(ca.make_config(384, 23), 4, 4)
     c = ca.default_config(4, 4, 1 << 28)
     assert ca.config_valid(c, 4, 4)
-    # a tile that does not fit in 160 KiB of LDS is not offered
-    assert not ca.config_valid(ca.make_config(512, 16), 8, 16)
+    # a tile whose keys alone do not fit in 160 KiB of LDS is not offered
+    assert not ca.config_valid(ca.make_config(1024, 32), 8, 16)
+    # keys and values staged in turn: a tile whose keys and values only fit one at a time
+    assert ca.config_valid(ca.make_config(1024, 16), 8, 4)
 
 
 def test_segmented_query_only():

@@ -230,6 +230,34 @@ def test_every_onesweep_shape(shape, n):
     assert_same(to_host(ko, np.uint32), to_host(vo, np.uint32), ek, ev)
 
 
+@pytest.mark.parametrize("case", ["u64_u32_window_desc", "u32_u64", "u64_u64"])
+@pytest.mark.parametrize("n", [8191, 300001, 1 << 21])
+def test_two_round_staging(case, n):
+    """1024 x 16 tiles whose keys and values do not fit in LDS together: the values are
+    staged in the keys' space after the keys have been written out."""
+    import cccl_amd as ca
+
+    rng = np.random.default_rng(n + len(case))
+    begin, end, desc = None, None, False
+    if case == "u64_u32_window_desc":
+        keys, vals, begin, end, desc = random_keys(rng, np.uint64, n, "and2"), np.arange(n, dtype=np.uint32), 0, 40, True
+    elif case == "u32_u64":
+        keys, vals = random_keys(rng, np.uint32, n, "and3"), rng.integers(0, 2**63, n, dtype=np.uint64)
+    else:
+        keys, vals = random_keys(rng, np.uint64, n), np.arange(n, dtype=np.uint64)
+    cfg = ca.make_config(1024, 16, single_tile_threads=0)
+    assert ca.config_valid(cfg, keys.dtype.itemsize, vals.dtype.itemsize)
+    tk = {np.uint32: torch.uint32, np.uint64: torch.uint64}
+    s = ca.Sorter(tk[keys.dtype.type], vals.dtype.itemsize, n, descending=desc, begin_bit=begin or 0, end_bit=end,
+                  config=cfg)
+    kd, vd = to_dev(keys), to_dev(vals)
+    ko, vo = torch.empty_like(kd), torch.empty_like(vd)
+    s.sort(kd, ko, vd, vo)
+    torch.cuda.synchronize()
+    ek, ev = oracle.sort(keys, vals, descending=desc, begin_bit=begin, end_bit=end)
+    assert_same(to_host(ko, keys.dtype), to_host(vo, vals.dtype), ek, ev)
+
+
 @pytest.mark.parametrize("shape", [(256, 16), (512, 8), (512, 16), (1024, 16)])
 @pytest.mark.parametrize("case", ["u32_pairs", "f32_desc", "u64_window", "i16_keys"])
 @pytest.mark.parametrize("n", [8191, 300000, 1 << 21])
