@@ -323,7 +323,9 @@ def leg_selector(torch, dev, calls: int):
             "regret_vs_exhaustive": round(j["regret_vs_exhaustive"], 4),
             "regret_excl_feature_time": round(j["regret_vs_exhaustive_excl_features"], 4),
             "selected_ms": round(j["selected_ms"], 3), "feature_ms": round(j["feature_ms"], 3),
-            "default_ms": round(j["default_ms"], 3), "exhaustive_ms": round(j["exhaustive_ms"], 3)}
+            "default_ms": round(j["default_ms"], 3), "exhaustive_ms": round(j["exhaustive_ms"], 3),
+            "library_default_ms": round(j.get("library_default_ms", float("nan")), 3),
+            "speedup_vs_library_default": round(j.get("speedup_vs_library_default", float("nan")), 3)}
 
 
 def main() -> int:

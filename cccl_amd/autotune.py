@@ -28,7 +28,7 @@ import numpy as np
 import torch
 
 from . import selector as S
-from .radix_sort import key_type_of
+from .radix_sort import SegmentedSorter, Sorter, key_type_of
 
 KEY_DTYPES = {"u8": torch.uint8, "u16": torch.uint16, "u32": torch.uint32, "u64": torch.uint64, "f32": torch.float32}
 ENTROPIES = ["uniform", "and2", "and3", "and4", "and5", "equal"]
@@ -169,8 +169,17 @@ def measure_call(call, buf: Buffers, features_out=None, verify=False):
             elif not torch.equal(ref, out):
                 raise AssertionError(f"arm {arm.name} gave a different permutation for call {call['id']}")
         del srt
+    # the library's own default configuration (what a caller gets without passing one)
+    if seg:
+        srt = SegmentedSorter(dtype, 4, n, nseg)
+        fn = lambda: srt.sort(kin, kout, sb, se, vin, vout)  # noqa: E731
+    else:
+        srt = Sorter(dtype, 4, n)
+        fn = lambda: srt.sort(kin, kout, vin, vout)  # noqa: E731
+    t_lib = time_fn(fn, reps_for(n))
+    del srt
     return {"call": call, "features": feat.cpu().tolist(), "feature_ms": t_feat, "times_ms": times,
-            "valid": valid, "num_segments": nseg}
+            "valid": valid, "num_segments": nseg, "library_default_ms": t_lib}
 
 
 def run_stream(calls, out_path, device="cuda", verify_every=50, log=sys.stdout):
@@ -221,11 +230,17 @@ def judge(records, choice) -> dict:
         tot_def += ts[S.FIXED_DEFAULT[kind]]
         tot_best += best
         worst = max(worst, t_sel / best - 1)
-    return {"calls": len(records), "selected_ms": tot_sel, "feature_ms": tot_feat, "default_ms": tot_def,
-            "exhaustive_ms": tot_best, "speedup_vs_default": tot_def / (tot_sel + tot_feat),
-            "speedup_vs_default_excl_features": tot_def / tot_sel,
-            "regret_vs_exhaustive": (tot_sel + tot_feat) / tot_best - 1,
-            "regret_vs_exhaustive_excl_features": tot_sel / tot_best - 1, "worst_call_regret": worst}
+    out = {"calls": len(records), "selected_ms": tot_sel, "feature_ms": tot_feat, "default_ms": tot_def,
+           "exhaustive_ms": tot_best, "speedup_vs_default": tot_def / (tot_sel + tot_feat),
+           "speedup_vs_default_excl_features": tot_def / tot_sel,
+           "regret_vs_exhaustive": (tot_sel + tot_feat) / tot_best - 1,
+           "regret_vs_exhaustive_excl_features": tot_sel / tot_best - 1, "worst_call_regret": worst}
+    if all("library_default_ms" in r for r in records):
+        # the size-dependent default the library picks when the caller passes no config
+        tot_lib = sum(r["library_default_ms"] for r in records)
+        out["library_default_ms"] = tot_lib
+        out["speedup_vs_library_default"] = tot_lib / (tot_sel + tot_feat)
+    return out
 
 
 def train_and_judge(records, train_frac=0.7, epochs=600, seed=0, device="cuda"):
@@ -264,6 +279,8 @@ def compact(data: dict) -> dict:
                      "feature_ms": float(f"{r['feature_ms']:.5g}"),
                      "times_ms": [None if t is None else float(f"{t:.5g}") for t in r["times_ms"]],
                      "valid": r["valid"], "num_segments": r["num_segments"]})
+        if "library_default_ms" in r:
+            recs[-1]["library_default_ms"] = float(f"{r['library_default_ms']:.5g}")
     return {"arms": data["arms"], "records": recs}
 
 

@@ -305,39 +305,59 @@ void twiddle_constants(int key_bytes, int key_kind, bool descending, uint64_t& x
 
 cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64_t num_items)
 {
+  // Chosen from the timings of a 2,000-call stream (n log-uniform in [2^10, 2^28], every
+  // key type, six entropies) and of 2^27-2^28-item sorts; where a tile does not fit in
+  // LDS for these key and value widths the next candidate is taken.
   cccl_amd_radix_sort_config_t c{};
   c.radix_bits            = kRadixBits;
   c.upsweep_blocks_per_cu = 4;
-  c.single_tile_threads   = 256;
-  c.single_tile_ipt       = 16;
+  c.algorithm             = CCCL_AMD_ALGO_PASSES;
   const int item_bytes    = key_bytes + val_bytes;
+  auto pick = [&](int threads, int ipt, int algorithm) {
+    c.onesweep_threads = threads;
+    c.onesweep_ipt     = ipt;
+    c.algorithm        = algorithm;
+    return config_is_valid(c, key_bytes, val_bytes);
+  };
+  // One workgroup sorts up to 8,192 items (4,096 where that tile does not fit).
+  c.single_tile_threads = 512;
+  c.single_tile_ipt     = 16;
+  if (!pick(256, 8, CCCL_AMD_ALGO_PASSES))
+  {
+    c.single_tile_threads = 256;
+  }
   if (num_items >= (uint64_t(1) << 22))
   {
-    // Timed at 2^27-2^28 items.  32-bit keys with 32-bit values: the largest tile
-    // (longest digit runs per tile, so the scatter writes fewer partial lines); pairs of
-    // 12 to 16 bytes: 1024 x 8; keys only and narrower pairs: 512 x 16 (two workgroups
-    // per CU hide the rank's latency better); wider pairs: 256 x 16.
-    if (key_bytes == 4 && val_bytes == 4)
+    // 32-bit keys with 32-bit values: the largest tile (longest digit runs per tile, so
+    // the scatter writes fewer partial lines); pairs of 12 to 16 bytes: 1024 x 8; keys
+    // only and narrower pairs: 512 x 16 (two workgroups per CU hide the rank's latency
+    // better); wider pairs: 256 x 16.
+    if (key_bytes == 4 && val_bytes == 4 && pick(1024, 16, CCCL_AMD_ALGO_PASSES))
     {
-      c.onesweep_threads = 1024;
-      c.onesweep_ipt     = 16;
+      return c;
     }
-    else if (val_bytes > 0 && item_bytes >= 12)
+    if (val_bytes > 0 && item_bytes >= 12 && item_bytes <= 16 && pick(1024, 8, CCCL_AMD_ALGO_PASSES))
     {
-      c.onesweep_threads = item_bytes > 16 ? 256 : 1024;
-      c.onesweep_ipt     = item_bytes > 16 ? 16 : 8;
+      return c;
     }
-    else
+    if (item_bytes <= 16 && pick(512, 16, CCCL_AMD_ALGO_PASSES))
     {
-      c.onesweep_threads = 512;
-      c.onesweep_ipt     = 16;
+      return c;
     }
+    pick(256, 16, CCCL_AMD_ALGO_PASSES);
+    return c;
   }
-  else
+  // Mid sizes: CUB's onesweep (one upfront histogram, then one kernel per pass: fewer
+  // launches than three per pass) up to 2^20 items; then 512 x 8 passes.
+  if (num_items >= (uint64_t(1) << 14) && num_items < (uint64_t(1) << 20) && pick(512, 8, CCCL_AMD_ALGO_ONESWEEP))
   {
-    c.onesweep_threads = 256;
-    c.onesweep_ipt     = 8;
+    return c;
   }
+  if (num_items >= (uint64_t(1) << 20) && pick(512, 8, CCCL_AMD_ALGO_PASSES))
+  {
+    return c;
+  }
+  pick(256, 8, CCCL_AMD_ALGO_PASSES);
   return c;
 }
 

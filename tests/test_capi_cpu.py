@@ -214,6 +214,19 @@ def test_configs_and_shapes():
     assert ca.config_valid(ca.make_config(1024, 16), 8, 4)
 
 
+def test_default_config_is_valid_everywhere():
+    """The default launch configuration exists for every key and value width and size."""
+    import cccl_amd as ca
+
+    for kb in (1, 2, 4, 8):
+        for vb in (0, 1, 2, 4, 8, 16):
+            for n in (1, 4096, 8192, 8193, 1 << 14, (1 << 20) - 1, 1 << 20, 1 << 22, 1 << 28, 1 << 31):
+                c = ca.default_config(kb, vb, n)
+                assert ca.config_valid(c, kb, vb), (kb, vb, n)
+                if n >= 1 << 20:
+                    assert c.algorithm == ca.ALGO_PASSES
+
+
 def test_segmented_query_only():
     import cccl_amd as ca
 

@@ -62,7 +62,7 @@ def assert_same(got_k, got_v, exp_k, exp_v):
         assert np.array_equal(got_v.view(np.uint8), exp_v.view(np.uint8)), "values differ"
 
 
-SIZES = [1, 7, 64, 1000, 4096, 4097, 65536, 100003, 1 << 20]
+SIZES = [1, 7, 64, 1000, 4096, 4097, 8192, 8193, 65536, 100003, 1 << 20]
 KEY_TYPES = [np.uint8, np.int8, np.uint16, np.int16, np.uint32, np.int32, np.uint64, np.int64, np.float16,
              np.float32, np.float64]
 
@@ -173,7 +173,7 @@ def test_float_nan_bit_pattern():
         assert_same(gk, None, ek, None)
 
 
-@pytest.mark.parametrize("n", [1, 100, 4096, 5000, 1 << 20])
+@pytest.mark.parametrize("n", [1, 100, 4096, 5000, 8192, 8193, 1 << 16, 1 << 20])
 @pytest.mark.parametrize("end_bit", [32, 24, 16, 8])
 def test_double_buffer_selector(n, end_bit):
     """catch2_test_device_radix_sort_keys.cu:418-447: result in d_keys.Current()."""
@@ -184,7 +184,10 @@ def test_double_buffer_selector(n, end_bit):
     ek, ev = oracle.sort(keys, vals, begin_bit=0, end_bit=end_bit)
     assert_same(gk, gv, ek, ev)
     passes = (end_bit + 7) // 8
-    single_tile = n <= 4096
+    import cccl_amd as ca
+
+    cfg = ca.default_config(4, 4, n)
+    single_tile = n <= cfg.single_tile_threads * cfg.single_tile_ipt  # one workgroup: result in the alternate
     assert sel == (1 if single_tile else passes & 1)
 
 
