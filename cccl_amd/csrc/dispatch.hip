@@ -313,30 +313,36 @@ cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64
   c.upsweep_blocks_per_cu = 4;
   c.algorithm             = CCCL_AMD_ALGO_PASSES;
   const int item_bytes    = key_bytes + val_bytes;
+  const uint64_t n        = num_items;
   auto pick = [&](int threads, int ipt, int algorithm) {
     c.onesweep_threads = threads;
     c.onesweep_ipt     = ipt;
     c.algorithm        = algorithm;
     return config_is_valid(c, key_bytes, val_bytes);
   };
-  // One workgroup sorts up to 8,192 items (4,096 where that tile does not fit).
-  c.single_tile_threads = 512;
+  // One workgroup sorts up to 8,192 items (256 x 16 up to 4,096, the faster shape there).
+  c.single_tile_threads = n <= 4096 ? 256 : 512;
   c.single_tile_ipt     = 16;
   if (!pick(256, 8, CCCL_AMD_ALGO_PASSES))
   {
     c.single_tile_threads = 256;
   }
-  if (num_items >= (uint64_t(1) << 22))
+  auto pow2 = [](int e) { return uint64_t(1) << e; };
+  if (n >= pow2(22))
   {
-    // 32-bit keys with 32-bit values: the largest tile (longest digit runs per tile, so
-    // the scatter writes fewer partial lines); pairs of 12 to 16 bytes: 1024 x 8; keys
-    // only and narrower pairs: 512 x 16 (two workgroups per CU hide the rank's latency
-    // better); wider pairs: 256 x 16.
-    if (key_bytes == 4 && val_bytes == 4 && pick(1024, 16, CCCL_AMD_ALGO_PASSES))
+    // 32-bit keys with 32-bit values from 2^26 items: the largest tile (longest digit
+    // runs per tile, so the scatter writes fewer partial lines); pairs of 12 to 16 bytes:
+    // 1024 x 8; otherwise 512 x 16 (512 x 8 below 2^24 items for 4-byte keys), where two
+    // workgroups per CU hide the rank's latency better; wider pairs: 256 x 16.
+    if (key_bytes == 4 && val_bytes == 4 && n >= pow2(26) && pick(1024, 16, CCCL_AMD_ALGO_PASSES))
     {
       return c;
     }
     if (val_bytes > 0 && item_bytes >= 12 && item_bytes <= 16 && pick(1024, 8, CCCL_AMD_ALGO_PASSES))
+    {
+      return c;
+    }
+    if (key_bytes == 4 && n < pow2(24) && pick(512, 8, CCCL_AMD_ALGO_PASSES))
     {
       return c;
     }
@@ -347,13 +353,14 @@ cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64
     pick(256, 16, CCCL_AMD_ALGO_PASSES);
     return c;
   }
-  // Mid sizes: CUB's onesweep (one upfront histogram, then one kernel per pass: fewer
-  // launches than three per pass) up to 2^20 items; then 512 x 8 passes.
-  if (num_items >= (uint64_t(1) << 14) && num_items < (uint64_t(1) << 20) && pick(512, 8, CCCL_AMD_ALGO_ONESWEEP))
+  // Mid sizes, keys of 4 or 8 bytes: CUB's onesweep (one upfront histogram, then one
+  // kernel per pass: fewer launches than three per pass) up to 2^20 items.  Keys of 1 or
+  // 2 bytes (one or two passes) gain nothing from the upfront histogram: 512 x 8 passes.
+  if (key_bytes >= 4 && n >= pow2(14) && n < pow2(20) && pick(512, 8, CCCL_AMD_ALGO_ONESWEEP))
   {
     return c;
   }
-  if (num_items >= (uint64_t(1) << 20) && pick(512, 8, CCCL_AMD_ALGO_PASSES))
+  if ((n >= pow2(20) || key_bytes <= 2) && n >= pow2(13) && pick(512, 8, CCCL_AMD_ALGO_PASSES))
   {
     return c;
   }
