@@ -5,17 +5,23 @@
 //   seg_init      zero the class counters and tile counters
 //   seg_classify  bin every segment by size into the one-workgroup classes or the large
 //                 class; large segments get a contiguous range of onesweep tiles
-//   seg_prep      per large segment: clear its digit counts and its tiles' first-pass
-//                 look-back words, map its tiles to it
-//   seg_histogram per large tile: digit counts of every pass into its segment's counts
-//   seg_starts    per large segment: the counts become digit starts within the segment
+//   seg_prep      per large segment: map its tiles to it; with look-back passes also
+//                 clear its digit counts and its tiles' first-pass look-back words
+//   seg_histogram per large tile (look-back passes): digit counts of every pass into
+//                 its segment's counts
+//   seg_starts    per large segment (look-back passes): the counts become digit starts
+//                 within the segment
 //   block_sort    one launch per one-workgroup class, one workgroup per segment
-//   onesweep      one batched launch per digit pass over every large segment's tiles
+//   per digit pass over every large segment's tiles, either one onesweep launch with
+//   look-back, or (table passes) seg_upsweep tile counts, seg_spine per-segment scans
+//   (tile prefixes and the segment's digit starts) and a downsweep
 // Pass count and final-buffer rules follow
 // cub/cub/device/dispatch/dispatch_segmented_radix_sort.cuh:678-893 with 8-bit digits:
 // with is_overwrite_okay the result ends in buf[(selector + passes) & 1], otherwise in
 // the alternate buffer (selector + 1).
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 
 #include "dispatch.hpp"
 #include "registry.hpp"
@@ -24,6 +30,27 @@
 namespace cccl_amd {
 
 namespace {
+
+// Workgroups of a kernel that fit on one CU at once (at least 1), cached per kernel.
+int resident_workgroups(const KernelEntry& k)
+{
+  static std::mutex mu;
+  static std::unordered_map<const void*, int> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(k.fn);
+  if (it != cache.end())
+  {
+    return it->second;
+  }
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k.fn, k.threads, k.lds_bytes) != hipSuccess || n < 1)
+  {
+    (void) hipGetLastError();
+    n = 1;
+  }
+  cache.emplace(k.fn, n);
+  return n;
+}
 
 template <class T>
 constexpr T ceil_div(T a, T b)
@@ -327,14 +354,25 @@ hipError_t segmented_radix_sort(
   }
   if (large_bound > 0)
   {
-    SegPrepParams pp{d_counters,          d_large_range, d_large_first, d_tile_slot, d_seg_bins,
-                     table_passes ? nullptr : d_lb[0], uint32_t(large_tile), passes};
+    SegPrepParams pp{d_counters,
+                     d_large_range,
+                     d_large_first,
+                     d_tile_slot,
+                     table_passes ? nullptr : d_seg_bins,
+                     table_passes ? nullptr : d_lb[0],
+                     uint32_t(large_tile),
+                     passes};
     if (hipError_t e =
           launch_kernel(reinterpret_cast<const void*>(&seg_prep_kernel), large_bound, 256, 0, stream, &pp, 4);
         e != hipSuccess)
     {
       return e;
     }
+  }
+  if (large_bound > 0 && !table_passes)
+  {
+    // Look-back passes: every pass's digit starts per segment up front (the table passes'
+    // spine computes them from the tile counts instead).
     SegHistogramParams hp{};
     hp.keys            = key_in;
     hp.counters        = d_counters;
@@ -378,8 +416,10 @@ hipError_t segmented_radix_sort(
     bp.float_mode     = float_mode;
     bp.begin_bit      = pr.begin_bit;
     bp.end_bit        = pr.end_bit;
-    const uint64_t per_cu = std::max<uint64_t>(1, (64 * 1024) / std::max<size_t>(cls[c]->lds_bytes, 1) + 1);
-    const uint64_t grid   = std::min<uint64_t>(uint64_t(num_segments), cus * std::min<uint64_t>(per_cu, 8));
+    // As many workgroups as fit on a CU at once (registers and LDS), at most 8: each
+    // walks its share of the class list, prefetching its next segment.
+    const uint64_t per_cu = uint64_t(std::min(resident_workgroups(*cls[c]), 8));
+    const uint64_t grid   = std::min<uint64_t>(uint64_t(num_segments), cus * per_cu);
     if (hipError_t e = launch_kernel(cls[c]->fn, grid, cls[c]->threads, cls[c]->lds_bytes, stream, &bp, 3);
         e != hipSuccess)
     {
@@ -412,7 +452,8 @@ hipError_t segmented_radix_sort(
       {
         return e;
       }
-      SegSpineParams sp{d_counters, d_large_range, d_large_first, uint32_t(large_tile), d_tile_counts};
+      SegSpineParams sp{d_counters,    d_large_range, d_large_first, uint32_t(large_tile),
+                        d_tile_counts, d_seg_bins,    passes,        pass};
       if (hipError_t e =
             launch_kernel(reinterpret_cast<const void*>(&seg_spine_kernel), large_bound, kRadix, 0, stream, &sp, 10);
           e != hipSuccess)

@@ -905,119 +905,141 @@ __global__ void __launch_bounds__(THREADS) block_sort_kernel(BlockSortParams p)
   const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
   const uint64_t num_segments = p.d_num_segments != nullptr ? uint64_t(*p.d_num_segments) : p.num_segments;
 
-  for (uint64_t s = blockIdx.x; s < num_segments; s += gridDim.x)
-  {
-    const uint64_t b = p.seg_range != nullptr ? uint64_t(p.seg_range[2 * s]) : 0;
+  // Segment s's items are in registers while the next segment of this workgroup is
+  // loaded, so its load latency hides behind the current sort.
+  uint64_t b = 0;
+  uint32_t m = 0;
+  UK keys[IPT];
+  VS vals[IPT];
+  auto load_segment = [&](uint64_t s, uint64_t& sb, uint32_t& sm, UK (&k)[IPT], VS (&v)[IPT]) {
+    sb               = p.seg_range != nullptr ? uint64_t(p.seg_range[2 * s]) : 0;
     const uint64_t e = p.seg_range != nullptr ? uint64_t(p.seg_range[2 * s + 1]) : p.num_items;
-    if (e <= b)
-    {
-      continue; // block-uniform
-    }
-    const uint32_t m = uint32_t(e - b);
-
-    const UK* kin = static_cast<const UK*>(p.keys_in) + b;
-    const VS* vin = static_cast<const VS*>(p.vals_in) + b;
-    UK keys[IPT];
-    VS vals[IPT];
+    sm               = e > sb ? uint32_t(e - sb) : 0u;
+    const UK* kin    = static_cast<const UK*>(p.keys_in) + sb;
+    const VS* vin    = static_cast<const VS*>(p.vals_in) + sb;
 #pragma unroll
     for (int i = 0; i < IPT; ++i)
     {
       const uint32_t j = strip + uint32_t(i * 64);
-      keys[i]          = j < m ? kin[j] : UK(0);
+      k[i]             = j < sm ? kin[j] : UK(0);
       if constexpr (kHasV)
       {
+        if (j < sm)
+        {
+          v[i] = vin[j];
+        }
+      }
+    }
+  };
+  if (uint64_t(blockIdx.x) < num_segments)
+  {
+    load_segment(blockIdx.x, b, m, keys, vals);
+  }
+
+  for (uint64_t s = blockIdx.x; s < num_segments; s += gridDim.x)
+  {
+    uint64_t nb = 0;
+    uint32_t nm = 0;
+    UK nkeys[IPT];
+    VS nvals[IPT];
+    if (s + gridDim.x < num_segments) // block-uniform
+    {
+      load_segment(s + gridDim.x, nb, nm, nkeys, nvals);
+    }
+    // Segments do not overlap, so the stores below never touch the prefetched items.
+    // Within the segment every thread has used all of its items (pass 0 ranks the keys
+    // and stages the values) before any thread passes pass 0's staging barrier, so the
+    // segment may be sorted in place.
+    if (m > 0) // block-uniform
+    {
+      for (int shift = p.begin_bit; shift < p.end_bit; shift += kRadixBits)
+      {
+        const int bits       = min(kRadixBits, p.end_bit - shift);
+        const uint32_t dmask = (1u << bits) - 1u;
+        uint32_t dig[IPT];
+#pragma unroll
+        for (int i = 0; i < IPT; ++i)
+        {
+          const uint32_t j = strip + uint32_t(i * 64);
+          dig[i]           = j < m ? extract_digit(tw(keys[i]), shift, dmask) : uint32_t(kRadix - 1);
+        }
+        zero_wave_counters(s_wcnt + wave * kRadix);
+
+        uint32_t rank[IPT];
+        wave_rank<IPT>(dig, rank, s_wcnt + wave * kRadix);
+        lds_barrier();
+
+        uint32_t total = 0;
+        if (tid < uint32_t(kRadix))
+        {
+#pragma unroll
+          for (int w = 0; w < kWaves; ++w)
+          {
+            const uint32_t c         = s_wcnt[w * kRadix + tid];
+            s_wcnt[w * kRadix + tid] = total;
+            total += c;
+          }
+        }
+        uint32_t all         = 0;
+        const uint32_t start = radix_exclusive_scan<uint32_t, true>(total, s_misc, all);
+        if (tid < uint32_t(kRadix))
+        {
+#pragma unroll
+          for (int w = 0; w < kWaves; ++w)
+          {
+            s_wcnt[w * kRadix + tid] += start;
+          }
+        }
+        lds_barrier();
+
+        const uint32_t* wc = s_wcnt + wave * kRadix;
+#pragma unroll
+        for (int i = 0; i < IPT; ++i)
+        {
+          const uint32_t pos = wc[dig[i]] + rank[i];
+          s_keys[pos]        = keys[i];
+          if constexpr (kHasV)
+          {
+            s_vals[pos] = vals[i];
+          }
+        }
+        lds_barrier();
+#pragma unroll
+        for (int i = 0; i < IPT; ++i)
+        {
+          const uint32_t j = strip + uint32_t(i * 64);
+          keys[i]          = s_keys[j];
+          if constexpr (kHasV)
+          {
+            vals[i] = s_vals[j];
+          }
+        }
+        lds_barrier();
+      }
+
+      UK* kout = static_cast<UK*>(p.keys_out) + b;
+      VS* vout = static_cast<VS*>(p.vals_out) + b;
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        const uint32_t j = strip + uint32_t(i * 64);
         if (j < m)
         {
-          vals[i] = vin[j];
+          kout[j] = keys[i];
+          if constexpr (kHasV)
+          {
+            vout[j] = vals[i];
+          }
         }
       }
     }
-    // Every item of the segment is in registers before the block writes anything.
-    __syncthreads();
-
-    for (int shift = p.begin_bit; shift < p.end_bit; shift += kRadixBits)
-    {
-      const int bits       = min(kRadixBits, p.end_bit - shift);
-      const uint32_t dmask = (1u << bits) - 1u;
-      uint32_t dig[IPT];
-#pragma unroll
-      for (int i = 0; i < IPT; ++i)
-      {
-        const uint32_t j = strip + uint32_t(i * 64);
-        dig[i]           = j < m ? extract_digit(tw(keys[i]), shift, dmask) : uint32_t(kRadix - 1);
-      }
-      for (uint32_t i = tid; i < uint32_t(kWaves * kRadix); i += THREADS)
-      {
-        s_wcnt[i] = 0;
-      }
-      __syncthreads();
-
-      uint32_t rank[IPT];
-      wave_rank<IPT>(dig, rank, s_wcnt + wave * kRadix);
-      __syncthreads();
-
-      uint32_t total = 0;
-      if (tid < uint32_t(kRadix))
-      {
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w)
-        {
-          const uint32_t c         = s_wcnt[w * kRadix + tid];
-          s_wcnt[w * kRadix + tid] = total;
-          total += c;
-        }
-      }
-      uint32_t all         = 0;
-      const uint32_t start = radix_exclusive_scan<uint32_t>(total, s_misc, all);
-      if (tid < uint32_t(kRadix))
-      {
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w)
-        {
-          s_wcnt[w * kRadix + tid] += start;
-        }
-      }
-      __syncthreads();
-
-      const uint32_t* wc = s_wcnt + wave * kRadix;
-#pragma unroll
-      for (int i = 0; i < IPT; ++i)
-      {
-        const uint32_t pos = wc[dig[i]] + rank[i];
-        s_keys[pos]        = keys[i];
-        if constexpr (kHasV)
-        {
-          s_vals[pos] = vals[i];
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < IPT; ++i)
-      {
-        const uint32_t j = strip + uint32_t(i * 64);
-        keys[i]          = s_keys[j];
-        if constexpr (kHasV)
-        {
-          vals[i] = s_vals[j];
-        }
-      }
-      __syncthreads();
-    }
-
-    UK* kout = static_cast<UK*>(p.keys_out) + b;
-    VS* vout = static_cast<VS*>(p.vals_out) + b;
+    b = nb;
+    m = nm;
 #pragma unroll
     for (int i = 0; i < IPT; ++i)
     {
-      const uint32_t j = strip + uint32_t(i * 64);
-      if (j < m)
-      {
-        kout[j] = keys[i];
-        if constexpr (kHasV)
-        {
-          vout[j] = vals[i];
-        }
-      }
+      keys[i] = nkeys[i];
+      vals[i] = nvals[i];
     }
   }
 }
@@ -1358,7 +1380,7 @@ struct SegPrepParams
   const int64_t* large_range;
   const uint32_t* large_first_tile;
   uint32_t* tile_slot;
-  uint32_t* seg_bins; // [slot][passes][kRadix]
+  uint32_t* seg_bins; // [slot][passes][kRadix], cleared (null: not accumulated, nothing to clear)
   uint32_t* lookback; // look-back buffer of the first pass: cleared for the slot's tiles (may be null)
   uint32_t large_tile;
   int num_passes;
@@ -1374,10 +1396,13 @@ static __global__ void __launch_bounds__(256) seg_prep_kernel(SegPrepParams p)
   {
     return;
   }
-  uint32_t* bins = p.seg_bins + uint64_t(slot) * p.num_passes * kRadix;
-  for (int i = threadIdx.x; i < p.num_passes * kRadix; i += blockDim.x)
+  if (p.seg_bins != nullptr) // counts accumulated by seg_histogram: start from zero
   {
-    bins[i] = 0;
+    uint32_t* bins = p.seg_bins + uint64_t(slot) * p.num_passes * kRadix;
+    for (int i = threadIdx.x; i < p.num_passes * kRadix; i += blockDim.x)
+    {
+      bins[i] = 0;
+    }
   }
   const uint64_t size  = uint64_t(p.large_range[2 * slot + 1] - p.large_range[2 * slot]);
   const uint32_t tiles = uint32_t((size + p.large_tile - 1) / p.large_tile);
@@ -1511,16 +1536,22 @@ struct SegSpineParams
   const uint32_t* slot_first_tile;
   uint32_t large_tile;
   uint32_t* tile_counts; // [tile][kRadix], replaced in place by the exclusive prefixes
+  uint32_t* seg_bins; // [slot][num_passes][kRadix]: this pass's digit starts within each segment
+  int num_passes;
+  int pass;
 };
 
 // One workgroup per large segment (grid = upper bound on their number), thread d owns
 // digit d: an exclusive scan of the segment's tile counts, read one 1 KiB row per tile
 // (coalesced), kSpineBatch rows in flight at a time.  Segments are independent, so the
-// segmented scan needs no carries between workgroups.
+// segmented scan needs no carries between workgroups.  The scan ends with the segment's
+// count of each digit; their exclusive scan over digits gives the digit starts within
+// the segment (so no separate histogram pass over the keys is needed).
 constexpr int kSpineBatch = 16;
 
 static __global__ void __launch_bounds__(kRadix) seg_spine_kernel(SegSpineParams p)
 {
+  __shared__ uint32_t s_wave[kRadix / 64];
   const uint32_t slot = blockIdx.x;
   if (slot >= p.counters[kSegLarge])
   {
@@ -1549,6 +1580,9 @@ static __global__ void __launch_bounds__(kRadix) seg_spine_kernel(SegSpineParams
       }
     }
   }
+  uint32_t all         = 0;
+  const uint32_t start = radix_exclusive_scan<uint32_t, true>(run, s_wave, all);
+  p.seg_bins[(uint64_t(slot) * p.num_passes + p.pass) * kRadix + d] = start;
 }
 
 struct SegHistogramParams
