@@ -120,8 +120,8 @@ class segmented_config(ctypes.Structure):
     """cccl_amd_segmented_config_t"""
     _fields_ = [
         ("num_block_classes", ctypes.c_int32),
-        ("block_threads", ctypes.c_int32 * 4),
-        ("block_ipt", ctypes.c_int32 * 4),
+        ("block_threads", ctypes.c_int32 * 6),
+        ("block_ipt", ctypes.c_int32 * 6),
         ("large_threads", ctypes.c_int32),
         ("large_ipt", ctypes.c_int32),
         ("strategy", ctypes.c_int32),

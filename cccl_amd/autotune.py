@@ -243,19 +243,35 @@ def judge(records, choice) -> dict:
     return out
 
 
-def train_and_judge(records, train_frac=0.7, epochs=600, seed=0, device="cuda"):
+def train_and_judge(records, train_frac=0.7, epochs=600, seed=0, device="cuda", epoch_choices=None,
+                    val_frac=0.15):
+    """Trains on the first train_frac of the stream and judges on the rest.  With
+    epoch_choices, the epoch count is picked on a validation slice (the last val_frac of
+    the training calls, networks trained on the calls before it), then the network is
+    trained on all training calls; the judged calls play no part in the choice."""
     n_train = int(len(records) * train_frac)
     X, T, M = dataset(records)
     X, T, M = X.to(device), T.to(device), M.to(device)
     tr, te = slice(0, n_train), slice(n_train, None)
+    valid = M > 0
+    chosen = {}
+    if epoch_choices:
+        n_fit = int(n_train * (1.0 - val_frac))
+        fit, va = slice(0, n_fit), slice(n_fit, n_train)
+        for e in epoch_choices:
+            cand = S.MLPSelector(seed=seed, device=device)
+            cand.fit(X[fit], T[fit], M[fit], epochs=e, seed=seed)
+            pick_va = cand.choose(X[va], valid[va]).cpu().tolist()
+            chosen[e] = judge(records[n_fit:n_train], pick_va)["regret_vs_exhaustive_excl_features"]
+        epochs = min(chosen, key=lambda e: (chosen[e], e))
     net = S.MLPSelector(seed=seed, device=device)
     untrained = S.MLPSelector(seed=seed, device=device)
     losses = net.fit(X[tr], T[tr], M[tr], epochs=epochs, seed=seed)
-    valid = M > 0
     pick = net.choose(X[te], valid[te]).cpu().tolist()
     pick_untrained = untrained.choose(X[te], valid[te]).cpu().tolist()
     held = records[n_train:]
-    out = {"train_calls": n_train, "test_calls": len(held), "losses": losses,
+    out = {"train_calls": n_train, "test_calls": len(held), "epochs": epochs,
+           "validation_regret_by_epochs": chosen, "losses": losses,
            "learned": judge(held, pick), "untrained": judge(held, pick_untrained)}
     # a hand rule a caller might write: our built-in default shapes
     return net, out
@@ -301,6 +317,8 @@ def main(argv=None) -> int:
     b.add_argument("--timings", required=True)
     b.add_argument("--weights", default=str(S.WEIGHTS_PATH))
     b.add_argument("--epochs", type=int, default=600)
+    b.add_argument("--epoch-choices", default="600,1500,3000",
+                   help="epoch counts to pick from on a validation slice of the training calls ('' = use --epochs)")
     b.add_argument("--report", default=None)
     args = ap.parse_args(argv)
     if args.cmd == "stream":
@@ -330,7 +348,8 @@ def main(argv=None) -> int:
     data = load_timings(args.timings)
     if data["arms"] != [x.name for x in S.ARMS]:
         raise SystemExit("timings were recorded for a different arm list")
-    net, rep = train_and_judge(data["records"], epochs=args.epochs)
+    choices = [int(e) for e in args.epoch_choices.split(",") if e.strip()]
+    net, rep = train_and_judge(data["records"], epochs=args.epochs, epoch_choices=choices or None)
     net.save(args.weights, meta={"trained_on": args.timings, "report": rep["learned"]})
     txt = json.dumps(rep, indent=1)
     if args.report:

@@ -112,23 +112,29 @@ __device__ void sample_entropies(const FeatureParams& p, uint32_t* s_hist, float
   }
 }
 
-// Item counts per segment-size bucket (<= 1Ki, (1Ki, 8Ki], (8Ki, 64Ki], > 64Ki), over
-// segments tid, tid + T, ...; kSegBatch begin/end pairs are loaded before any is used.
+// Item counts per segment-size bucket (<= 1Ki, (1Ki, 8Ki], (8Ki, 64Ki], > 64Ki) over a
+// strided sample of at most kSegSample segments (segment k * S / m of S = min(S, m)):
+// the shares of items per bucket are the feature, so a sample estimates them in one
+// round of loads (each thread has its kSegBatch begin/end pairs in flight at once).
+constexpr int64_t kSegSample = int64_t(kFeatThreads) * kSegBatch;
+
 template <class OffsetT>
 __device__ void segment_mix(const FeatureParams& p, uint64_t (&acc)[4])
 {
-  const OffsetT* bo  = static_cast<const OffsetT*>(p.begin_offsets);
-  const OffsetT* eo  = static_cast<const OffsetT*>(p.end_offsets);
-  const int64_t step = int64_t(kFeatThreads) * kSegBatch;
-  for (int64_t s0 = threadIdx.x; s0 < p.num_segments; s0 += step)
+  const OffsetT* bo = static_cast<const OffsetT*>(p.begin_offsets);
+  const OffsetT* eo = static_cast<const OffsetT*>(p.end_offsets);
+  const int64_t m   = p.num_segments;
+  const int64_t S   = m < kSegSample ? m : kSegSample;
   {
+    const int64_t s0 = threadIdx.x;
     OffsetT b[kSegBatch], e[kSegBatch];
 #pragma unroll
     for (int j = 0; j < kSegBatch; ++j)
     {
-      const int64_t s = s0 + int64_t(j) * kFeatThreads;
-      b[j]            = s < p.num_segments ? bo[s] : OffsetT(0);
-      e[j]            = s < p.num_segments ? eo[s] : OffsetT(0);
+      const int64_t k = s0 + int64_t(j) * kFeatThreads;
+      const int64_t s = k < S ? int64_t(uint64_t(k) * uint64_t(m) / uint64_t(S)) : 0; // < m; k * m < 2^44
+      b[j]            = k < S ? bo[s] : OffsetT(0);
+      e[j]            = k < S ? eo[s] : OffsetT(0);
     }
 #pragma unroll
     for (int j = 0; j < kSegBatch; ++j)

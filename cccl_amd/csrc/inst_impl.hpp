@@ -82,14 +82,26 @@ KernelEntry seg_upsweep_entry()
 template <class UK, class V, int T, int I>
 KernelEntry block_sort_entry()
 {
-  using L = BlockSortLayout<UK, V, T, I>;
-  if constexpr (L::kBytes <= kMaxLdsBytes)
+  if constexpr (T == 64)
   {
-    return {int(sizeof(UK)), value_bytes<V>(), T, I, reinterpret_cast<const void*>(&block_sort_kernel<UK, V, T, I>), L::kBytes};
+    // 64 threads per segment: the wave sort (launched with up to kWaveSortThreads
+    // threads, one segment per wave)
+    using W = WaveSortLayout<UK, V, I>;
+    return {int(sizeof(UK)), value_bytes<V>(), T, I, reinterpret_cast<const void*>(&wave_sort_kernel<UK, V, I>),
+            W::kBytes};
   }
   else
   {
-    return {int(sizeof(UK)), value_bytes<V>(), T, I, nullptr, L::kBytes};
+    using L = BlockSortLayout<UK, V, T, I>;
+    if constexpr (L::kBytes <= kMaxLdsBytes)
+    {
+      return {int(sizeof(UK)), value_bytes<V>(), T, I, reinterpret_cast<const void*>(&block_sort_kernel<UK, V, T, I>),
+              L::kBytes};
+    }
+    else
+    {
+      return {int(sizeof(UK)), value_bytes<V>(), T, I, nullptr, L::kBytes};
+    }
   }
 }
 

@@ -19,8 +19,8 @@ namespace mlp {
 
 constexpr int kIn      = 16;
 constexpr int kHidden  = 64;
-constexpr int kMaxOut  = 64;
-constexpr int kSamples = 32; // samples per workgroup (keeps the LDS tiles at 43 KiB)
+constexpr int kMaxOut  = 128;
+constexpr int kSamples = 32; // samples per workgroup (keeps the LDS tiles at 51 KiB)
 constexpr int kThreads = 256;
 
 __host__ __device__ constexpr int param_count(int out)

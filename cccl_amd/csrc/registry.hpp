@@ -69,8 +69,11 @@ const KeyWidthTable* kernels_for_key_bytes(int key_bytes);
   X(512, 16)                            \
   X(1024, 16)
 
-// Launch shapes compiled for the single-workgroup sort.
+// Launch shapes compiled for the single-workgroup sort (64 threads: one wave per
+// segment, the wave sort).
 #define CCCL_AMD_BLOCK_SORT_SHAPES(X) \
+  X(64, 1)                            \
+  X(64, 4)                            \
   X(256, 4)                           \
   X(256, 8)                           \
   X(256, 16)                          \

@@ -32,7 +32,7 @@ namespace cccl_amd {
 namespace {
 
 // Workgroups of a kernel that fit on one CU at once (at least 1), cached per kernel.
-int resident_workgroups(const KernelEntry& k)
+int resident_workgroups(const KernelEntry& k, int threads)
 {
   static std::mutex mu;
   static std::unordered_map<const void*, int> cache;
@@ -43,7 +43,7 @@ int resident_workgroups(const KernelEntry& k)
     return it->second;
   }
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k.fn, k.threads, k.lds_bytes) != hipSuccess || n < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k.fn, threads, k.lds_bytes) != hipSuccess || n < 1)
   {
     (void) hipGetLastError();
     n = 1;
@@ -418,9 +418,12 @@ hipError_t segmented_radix_sort(
     bp.end_bit        = pr.end_bit;
     // As many workgroups as fit on a CU at once (registers and LDS), at most 8: each
     // walks its share of the class list, prefetching its next segment.
-    const uint64_t per_cu = uint64_t(std::min(resident_workgroups(*cls[c]), 8));
-    const uint64_t grid   = std::min<uint64_t>(uint64_t(num_segments), cus * per_cu);
-    if (hipError_t e = launch_kernel(cls[c]->fn, grid, cls[c]->threads, cls[c]->lds_bytes, stream, &bp, 3);
+    // A 64-thread class is the wave sort: kWaveSortWaves segments per workgroup.
+    const int threads        = cls[c]->threads == 64 ? kWaveSortThreads : cls[c]->threads;
+    const uint64_t seg_per_wg = uint64_t(cls[c]->threads == 64 ? kWaveSortWaves : 1);
+    const uint64_t per_cu     = uint64_t(std::min(resident_workgroups(*cls[c], threads), 8));
+    const uint64_t grid = std::min<uint64_t>(ceil_div<uint64_t>(uint64_t(num_segments), seg_per_wg), cus * per_cu);
+    if (hipError_t e = launch_kernel(cls[c]->fn, grid, threads, cls[c]->lds_bytes, stream, &bp, 3);
         e != hipSuccess)
     {
       return e;
@@ -547,22 +550,21 @@ extern "C" CCCL_C_API void cccl_amd_segmented_default_config(int key_bytes, int 
   {
     return;
   }
-  *cfg                   = cccl_amd_segmented_config_t{};
-  const int item         = key_bytes + value_bytes;
-  cfg->num_block_classes = 4;
-  cfg->block_threads[0]  = 256;
-  cfg->block_ipt[0]      = 4;
-  cfg->block_threads[1]  = 256;
-  cfg->block_ipt[1]      = 8;
-  cfg->block_threads[2]  = 256;
-  cfg->block_ipt[2]      = 16;
-  cfg->block_threads[3]  = 512;
-  cfg->block_ipt[3]      = 16;
-  if (item > 16)
+  *cfg           = cccl_amd_segmented_config_t{};
+  const int item = key_bytes + value_bytes;
+  // Size classes: one wave per segment up to 64 and 256 items, then one workgroup per
+  // segment up to 1,024 / 2,048 / 4,096 / 8,192 items (the last only where its tile fits
+  // in LDS); larger segments go through the batched digit passes with 512 x 16 tiles
+  // (256 x 16 for items above 12 bytes).  Timed over lognormal batches and the 2,000-call
+  // stream.
+  const int shapes[6][2] = {{64, 1}, {64, 4}, {256, 4}, {256, 8}, {256, 16}, {512, 16}};
+  cfg->num_block_classes = item > 16 ? 5 : 6;
+  for (int c = 0; c < cfg->num_block_classes; ++c)
   {
-    cfg->num_block_classes = 3;
+    cfg->block_threads[c] = shapes[c][0];
+    cfg->block_ipt[c]     = shapes[c][1];
   }
-  cfg->large_threads = item > 12 ? 256 : (item <= 8 ? 1024 : 512);
+  cfg->large_threads = item > 12 ? 256 : 512;
   cfg->large_ipt     = 16;
   cfg->strategy      = 2; // table passes: no look-back chains between a segment's tiles
 }

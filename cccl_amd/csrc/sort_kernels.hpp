@@ -1045,6 +1045,186 @@ __global__ void __launch_bounds__(THREADS) block_sort_kernel(BlockSortParams p)
 }
 
 // ---------------------------------------------------------------------------
+// One wave per segment of up to 64 * IPT items: the whole LSD sort stays inside the
+// wave (its own digit counters, its own LDS staging area), so no workgroup barrier is
+// needed and a workgroup's waves sort different segments side by side.  Tiny segments
+// (the bulk of a lognormal batch by count) then cost one wave instead of a padded
+// workgroup tile.  The launch may use any multiple of 64 threads, at most
+// kWaveSortWaves waves; with one wave and a null seg_range it sorts [0, num_items).
+// ---------------------------------------------------------------------------
+
+constexpr int kWaveSortWaves   = 4;
+constexpr int kWaveSortThreads = 64 * kWaveSortWaves;
+
+template <class UK, class V, int IPT>
+struct WaveSortLayout
+{
+  static constexpr int kItems      = 64 * IPT;
+  static constexpr bool kHasV      = !std::is_same<V, NoValue>::value;
+  static constexpr size_t kCnt     = size_t(kRadix) * 4;
+  static constexpr size_t kOffKeys = kCnt;
+  static constexpr size_t kOffVals = (kOffKeys + size_t(kItems) * sizeof(UK) + 15) & ~size_t(15);
+  static constexpr size_t kPerWave = (kOffVals + (kHasV ? size_t(kItems) * sizeof(V) : 0) + 15) & ~size_t(15);
+  static constexpr size_t kBytes   = kPerWave * kWaveSortWaves;
+};
+
+// Orders one wave's LDS accesses across its lanes: a wave's LDS operations execute in
+// program order, so it is enough to keep the compiler from moving them across.
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+template <class UK, class V, int IPT>
+__global__ void __launch_bounds__(kWaveSortThreads) wave_sort_kernel(BlockSortParams p)
+{
+  using L              = WaveSortLayout<UK, V, IPT>;
+  constexpr bool kHasV = L::kHasV;
+  using VS             = typename std::conditional<kHasV, V, uint8_t>::type;
+
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = threadIdx.x >> 6;
+  unsigned char* mine = smem + wave * L::kPerWave;
+  uint32_t* wc        = reinterpret_cast<uint32_t*>(mine);
+  UK* s_keys          = reinterpret_cast<UK*>(mine + L::kOffKeys);
+  VS* s_vals          = reinterpret_cast<VS*>(mine + L::kOffVals);
+  const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
+  const uint64_t num_segments = p.d_num_segments != nullptr ? uint64_t(*p.d_num_segments) : p.num_segments;
+  const uint64_t waves        = blockDim.x >> 6;
+  const uint64_t stride       = uint64_t(gridDim.x) * waves;
+
+  // Item i of lane l is segment position i * 64 + l (so loads and stores coalesce, and
+  // wave_rank's slot-by-slot, lane-by-lane order is position order: stable).
+  uint64_t b = 0;
+  uint32_t m = 0;
+  UK keys[IPT];
+  VS vals[IPT];
+  auto load_segment = [&](uint64_t s, uint64_t& sb, uint32_t& sm, UK (&k)[IPT], VS (&v)[IPT]) {
+    sb               = p.seg_range != nullptr ? uint64_t(p.seg_range[2 * s]) : 0;
+    const uint64_t e = p.seg_range != nullptr ? uint64_t(p.seg_range[2 * s + 1]) : p.num_items;
+    sm               = e > sb ? uint32_t(e - sb) : 0u;
+    const UK* kin    = static_cast<const UK*>(p.keys_in) + sb;
+    const VS* vin    = static_cast<const VS*>(p.vals_in) + sb;
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      const uint32_t j = uint32_t(i * 64) + lane;
+      k[i]             = j < sm ? kin[j] : UK(0);
+      if constexpr (kHasV)
+      {
+        if (j < sm)
+        {
+          v[i] = vin[j];
+        }
+      }
+    }
+  };
+  uint64_t s = uint64_t(blockIdx.x) * waves + wave;
+  if (s < num_segments)
+  {
+    load_segment(s, b, m, keys, vals);
+  }
+  for (; s < num_segments; s += stride) // wave-uniform
+  {
+    uint64_t nb = 0;
+    uint32_t nm = 0;
+    UK nkeys[IPT];
+    VS nvals[IPT];
+    if (s + stride < num_segments)
+    {
+      load_segment(s + stride, nb, nm, nkeys, nvals);
+    }
+    // One wave reads the whole segment before writing any of it: in-place is safe.
+    if (m > 0)
+    {
+      for (int shift = p.begin_bit; shift < p.end_bit; shift += kRadixBits)
+      {
+        const int bits       = min(kRadixBits, p.end_bit - shift);
+        const uint32_t dmask = (1u << bits) - 1u;
+        uint32_t dig[IPT];
+#pragma unroll
+        for (int i = 0; i < IPT; ++i)
+        {
+          const uint32_t j = uint32_t(i * 64) + lane;
+          dig[i]           = j < m ? extract_digit(tw(keys[i]), shift, dmask) : uint32_t(kRadix - 1);
+        }
+        zero_wave_counters(wc);
+        uint32_t rank[IPT];
+        wave_rank<IPT>(dig, rank, wc);
+        // Exclusive scan of the wave's counters: lane l owns digits 4l .. 4l+3.
+        uint32_t c[kRadix / 64];
+#pragma unroll
+        for (int q = 0; q < kRadix / 64; ++q)
+        {
+          c[q] = lds_load_wave(wc + (kRadix / 64) * lane + q);
+        }
+        uint32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < kRadix / 64; ++q)
+        {
+          sum += c[q];
+        }
+        uint32_t run = wave_inclusive_scan<uint32_t>(sum) - sum;
+#pragma unroll
+        for (int q = 0; q < kRadix / 64; ++q)
+        {
+          lds_store_wave(wc + (kRadix / 64) * lane + q, run);
+          run += c[q];
+        }
+#pragma unroll
+        for (int i = 0; i < IPT; ++i)
+        {
+          const uint32_t pos = lds_load_wave(wc + dig[i]) + rank[i];
+          s_keys[pos]        = keys[i];
+          if constexpr (kHasV)
+          {
+            s_vals[pos] = vals[i];
+          }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < IPT; ++i)
+        {
+          const uint32_t j = uint32_t(i * 64) + lane;
+          keys[i]          = s_keys[j];
+          if constexpr (kHasV)
+          {
+            vals[i] = s_vals[j];
+          }
+        }
+        wave_lds_sync();
+      }
+      UK* kout = static_cast<UK*>(p.keys_out) + b;
+      VS* vout = static_cast<VS*>(p.vals_out) + b;
+#pragma unroll
+      for (int i = 0; i < IPT; ++i)
+      {
+        const uint32_t j = uint32_t(i * 64) + lane;
+        if (j < m)
+        {
+          kout[j] = keys[i];
+          if constexpr (kHasV)
+          {
+            vout[j] = vals[i];
+          }
+        }
+      }
+    }
+    b = nb;
+    m = nm;
+#pragma unroll
+    for (int i = 0; i < IPT; ++i)
+    {
+      keys[i] = nkeys[i];
+      vals[i] = nvals[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // One workgroup per segment and digit pass, looping over the segment's tiles: the
 // schedule of cub::DeviceSegmentedRadixSort (kernel_segmented_radix_sort.cuh:100-291:
 // digit counts over the whole segment, a block scan, then a tile-by-tile stable
@@ -1250,7 +1430,7 @@ __global__ void __launch_bounds__(THREADS) seg_pass_kernel(SegPassParams p)
 // Segments are binned by size into up to kMaxBlockClasses one-workgroup classes
 // (class c takes sizes in (max[c-1], max[c]]) and one large class (sizes above the last
 // class max) that the batched onesweep sorts.
-constexpr int kMaxBlockClasses = 4;
+constexpr int kMaxBlockClasses = 6;
 
 enum SegCounter : int
 {

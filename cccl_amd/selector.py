@@ -71,6 +71,7 @@ def _plain_arms() -> list[Arm]:
 def _segmented_arms() -> list[Arm]:
     arms = []
     class_sets = {
+        "w6": ((64, 1), (64, 4), (256, 4), (256, 8), (256, 16), (512, 16)),
         "c4": ((256, 4), (256, 8), (256, 16), (512, 16)),
         "c2": ((256, 8), (512, 16)),
         "c1": ((256, 16),),

@@ -39,7 +39,7 @@ CCCL_C_API int cccl_amd_sort_features(
 
 /* Config-selector network 16 -> 64 -> 64 -> out (ReLU), fp32 parameters in one array of
  * cccl_amd_mlp_param_count(out) floats: W1[16][64], b1[64], W2[64][64], b2[64],
- * W3[64][out], b3[out].  out <= 64. */
+ * W3[64][out], b3[out].  out <= 128. */
 CCCL_C_API int cccl_amd_mlp_param_count(int out);
 
 /* Bytes of the per-call workspace cccl_amd_mlp_train_step needs for `batch` samples. */

@@ -56,9 +56,9 @@ typedef struct cccl_amd_radix_sort_config_t
  * are ignored). */
 typedef struct cccl_amd_segmented_config_t
 {
-  int32_t num_block_classes; /* 0..4 */
-  int32_t block_threads[4];
-  int32_t block_ipt[4];
+  int32_t num_block_classes; /* 0..6 */
+  int32_t block_threads[6]; /* 64: one wave per segment (the wave sort) */
+  int32_t block_ipt[6];
   int32_t large_threads;
   int32_t large_ipt;
   int32_t strategy; /* 0 size-binned batch with look-back, 2 size-binned batch with table passes,

@@ -188,6 +188,8 @@ def test_per_segment_pass_schedule(large):
     ([(256, 8), (512, 16)], (512, 8)),
     ([(256, 4), (256, 8), (256, 16), (512, 16)], (512, 16)),
     ([(256, 8), (512, 16)], (1024, 16)),
+    ([(64, 1), (64, 4), (256, 4), (256, 8), (256, 16), (512, 16)], (512, 16)),  # wave sort classes
+    ([(64, 4), (256, 16)], (256, 16)),
 ])
 def test_class_routings(classes, large, strategy):
     """Look-back (strategy 0) and table passes (strategy 2) for the large class."""
@@ -234,3 +236,23 @@ def test_many_tiny_segments():
     keys = random_keys(rng, np.uint16, n)
     vals = np.arange(n, dtype=np.uint32)
     check(keys, vals, sb, se, descending=True)
+
+
+@pytest.mark.parametrize("dtype,vdtype", [(np.uint32, np.uint32), (np.uint16, None), (np.float64, np.uint64),
+                                          (np.int8, np.uint8)])
+def test_wave_sort_classes(dtype, vdtype):
+    """Segments of up to 64 and 256 items sorted one per wave (several per workgroup),
+    sizes 0..300 so every wave-class boundary and empty segments occur, in place too."""
+    rng = np.random.default_rng(57 + np.dtype(dtype).itemsize)
+    sizes = rng.integers(0, 301, 20000)
+    sizes[::97] = 64
+    sizes[::89] = 65
+    sizes[::83] = 256
+    sb, se = contiguous_offsets(sizes)
+    n = int(se[-1])
+    keys = random_keys(rng, dtype, n, "and2")
+    vals = None if vdtype is None else rng.integers(0, np.iinfo(vdtype).max, n, dtype=vdtype)
+    cfg = _cfg([(64, 1), (64, 4), (256, 16)], (256, 16), 2)
+    check(keys, vals, sb, se, config=cfg)
+    check(keys, vals, sb, se, descending=True, overwrite=True, config=cfg)
+

@@ -25,7 +25,7 @@ def _net(out, seed=1):
     return MLPSelector(num_out=out, seed=seed)
 
 
-@pytest.mark.parametrize("batch,out", [(1, 1), (37, 5), (300, 31), (1000, 64)])
+@pytest.mark.parametrize("batch,out", [(1, 1), (37, 5), (300, 31), (1000, 64), (500, 70), (77, 128)])
 def test_forward_matches_cpu(batch, out):
     rng = np.random.default_rng(batch)
     net = _net(out)
@@ -35,7 +35,7 @@ def test_forward_matches_cpu(batch, out):
     assert np.allclose(y, yr, rtol=1e-4, atol=1e-4)
 
 
-@pytest.mark.parametrize("batch,out", [(33, 3), (257, 29), (1024, 64)])
+@pytest.mark.parametrize("batch,out", [(33, 3), (257, 29), (1024, 64), (300, 70), (64, 128)])
 def test_gradient_matches_cpu_float64(batch, out):
     """fp32 device gradient vs float64 backprop (pinned by finite differences): relative
     error of the whole gradient vector below 1e-4."""
@@ -114,3 +114,26 @@ def test_features_match_host_statistics():
     assert abs(f[12] - h) < 0.05
     f2 = S.features(kd, n, key_type_of(torch.uint32), 0, 0, 32, True).cpu().numpy()
     assert f2[6] == 0.0 and f2[4] == 1.0 and f2[11] == 1.0
+
+
+def test_segment_mix_is_sampled_for_many_segments():
+    """Above 8,192 segments the size-bucket shares come from segments k * m / 8192."""
+    from cccl_amd import selector as S
+    from cccl_amd.radix_sort import key_type_of
+
+    rng = np.random.default_rng(9)
+    n = 1 << 20
+    kd = to_dev(rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32))
+    m = 50_000
+    sizes = np.exp(rng.normal(np.log(700), 2.0, m)).astype(np.int64).clip(1, 300_000)
+    ends = np.cumsum(sizes)
+    begins = ends - sizes
+    f = S.features(kd, n, key_type_of(torch.uint32), 4, 0, 32, False, to_dev(begins), to_dev(ends), m).cpu().numpy()
+
+    def shares(sz):
+        edges = [(0, 1024), (1024, 8192), (8192, 65536), (65536, 1 << 62)]
+        return np.array([sz[(sz > lo) & (sz <= hi)].sum() for lo, hi in edges], np.float64) / sz.sum()
+
+    sample = sizes[(np.arange(8192, dtype=np.int64) * m) // 8192]
+    assert np.allclose(f[8:12], shares(sample), atol=1e-6)
+    assert np.allclose(f[8:12], shares(sizes), atol=0.08)  # a sample of one in six

@@ -233,6 +233,27 @@ def test_every_onesweep_shape(shape, n):
     assert_same(to_host(ko, np.uint32), to_host(vo, np.uint32), ek, ev)
 
 
+@pytest.mark.parametrize("single", [(64, 1), (64, 4)])
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 200, 256])
+def test_wave_sort_single_tile(single, n):
+    """The wave sort as the one-workgroup sort of a plain call (one wave sorts it all)."""
+    import cccl_amd as ca
+
+    if n > single[0] * single[1]:
+        pytest.skip("above this single-tile size")
+    rng = np.random.default_rng(n + single[1])
+    keys = random_keys(rng, np.uint32, n, "and2")
+    vals = np.arange(n, dtype=np.uint32)
+    cfg = ca.make_config(256, 8, single_tile_threads=single[0], single_tile_ipt=single[1])
+    s = ca.Sorter(torch.uint32, 4, n, config=cfg)
+    kd, vd = to_dev(keys), to_dev(vals)
+    ko, vo = torch.empty_like(kd), torch.empty_like(vd)
+    s.sort(kd, ko, vd, vo)
+    torch.cuda.synchronize()
+    ek, ev = oracle.sort(keys, vals)
+    assert_same(to_host(ko, np.uint32), to_host(vo, np.uint32), ek, ev)
+
+
 @pytest.mark.parametrize("case", ["u64_u32_window_desc", "u32_u64", "u64_u64"])
 @pytest.mark.parametrize("n", [8191, 300001, 1 << 21])
 def test_two_round_staging(case, n):
