@@ -250,6 +250,13 @@ def leg_segmented(torch, ca, dev, steps):
     graph_ok = bool(torch.equal(kout, ref_k) and torch.equal(vout, ref_v))  # replays give the eager result
     del g, s
 
+    # the library's default segmented configuration (what a caller gets without one)
+    from cccl_amd.radix_sort import SegmentedSorter
+
+    s = SegmentedSorter(torch.uint32, 4, n, nseg)
+    ms_default = time_steps(torch, lambda: s.sort(kin, kout, begins, ends, vin, vout), steps, 1)
+    del s
+
     # CUB's DeviceSegmentedRadixSort schedule: one workgroup per segment per pass
     cub_arm = S.FIXED_DEFAULT["segmented"]
     s, fn = run(S.ARMS[cub_arm])
@@ -287,6 +294,7 @@ def leg_segmented(torch, ca, dev, steps):
         "autotuned_config": S.ARMS[best].name,
         "batch_ms": round(ms_batch, 4), "batch_gkeys_per_s": round(n / ms_batch / 1e6, 3),
         "hipgraph_replay_ms": round(ms_graph, 4), "hipgraph_replay_ok": graph_ok,
+        "library_default_ms": round(ms_default, 4),
         "cub_schedule_ms": round(ms_cub, 4), "cub_schedule_config": S.ARMS[cub_arm].name,
         "per_segment_loop_ms_scaled": round(ms_loop, 2),
         "per_segment_loop_sample": f"{sample.size} random segments timed ({ms_loop_sample:.2f} ms), x {nseg / sample.size:.0f}",

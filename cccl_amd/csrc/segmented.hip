@@ -6,9 +6,9 @@
 //   seg_classify  bin every segment by size into the one-workgroup classes or the large
 //                 class; large segments get a contiguous range of onesweep tiles
 //   seg_prep      per large segment: map its tiles to it; with look-back passes also
-//                 clear its digit counts and its tiles' first-pass look-back words
-//   seg_histogram per large tile (look-back passes): digit counts of every pass into
-//                 its segment's counts
+//                 clear its digit counts
+//   seg_histogram per large tile (look-back passes): clear the tile's first-pass
+//                 look-back words; digit counts of every pass into its segment's counts
 //   seg_starts    per large segment (look-back passes): the counts become digit starts
 //                 within the segment
 //   block_sort    one launch per one-workgroup class, one workgroup per segment
@@ -359,7 +359,7 @@ hipError_t segmented_radix_sort(
                      d_large_first,
                      d_tile_slot,
                      table_passes ? nullptr : d_seg_bins,
-                     table_passes ? nullptr : d_lb[0],
+                     nullptr, // first-pass look-back words: cleared per tile by seg_histogram
                      uint32_t(large_tile),
                      passes};
     if (hipError_t e =
@@ -380,6 +380,7 @@ hipError_t segmented_radix_sort(
     hp.slot_first_tile = d_large_first;
     hp.slot_range      = d_large_range;
     hp.seg_bins        = d_seg_bins;
+    hp.lookback        = d_lb[0];
     hp.xor_mask        = xor_mask;
     hp.large_tile      = uint32_t(large_tile);
     hp.float_mode      = float_mode;

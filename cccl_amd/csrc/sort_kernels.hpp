@@ -1567,8 +1567,8 @@ struct SegPrepParams
 };
 
 // One workgroup per large-segment slot (the grid is an upper bound on their number):
-// clears the slot's digit counts and its tiles' first-pass look-back words, and writes
-// the slot id of each of its tiles.
+// clears the slot's digit counts (and, if given, its tiles' first-pass look-back words),
+// and writes the slot id of each of its tiles.
 static __global__ void __launch_bounds__(256) seg_prep_kernel(SegPrepParams p)
 {
   const uint32_t slot = blockIdx.x;
@@ -1773,6 +1773,7 @@ struct SegHistogramParams
   const uint32_t* slot_first_tile;
   const int64_t* slot_range;
   uint32_t* seg_bins;
+  uint32_t* lookback; // first pass's look-back words: each tile clears its own (may be null)
   uint64_t xor_mask;
   uint32_t large_tile;
   int float_mode;
@@ -1782,7 +1783,9 @@ struct SegHistogramParams
 };
 
 // One workgroup per large tile (grid = upper bound): digit counts of every pass for the
-// tile's items, added into its segment's counts.
+// tile's items, added into its segment's counts.  Each thread has kHistBatch key loads in
+// flight before it counts them.
+constexpr int kHistBatch = 16;
 template <class UK>
 __global__ void __launch_bounds__(kHistThreads) seg_histogram_kernel(SegHistogramParams p)
 {
@@ -1804,17 +1807,38 @@ __global__ void __launch_bounds__(kHistThreads) seg_histogram_kernel(SegHistogra
   const UK* keys         = static_cast<const UK*>(p.keys);
   const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
   const uint32_t part = threadIdx.x % kHistParts;
-  for (uint64_t i = b + threadIdx.x; i < e; i += kHistThreads)
+  if (p.lookback != nullptr)
   {
-    const UK t = tw(keys[i]);
-    for (int pass = 0; pass < p.num_passes; ++pass)
+    for (uint32_t i = threadIdx.x; i < uint32_t(kRadix); i += kHistThreads)
     {
-      const int shift     = p.begin_bit + pass * kRadixBits;
-      const int bits      = min(kRadixBits, p.end_bit - shift);
-      const uint32_t mask = bits > 0 ? (1u << bits) - 1u : 0u;
-      const uint32_t d    = extract_digit(t, shift, mask);
-      __hip_atomic_fetch_add(
-        &s_hist[(pass * kRadix + d) * kHistParts + part], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      p.lookback[uint64_t(tile) * kRadix + i] = 0u;
+    }
+  }
+  for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += uint64_t(kHistThreads) * kHistBatch)
+  {
+    UK k[kHistBatch];
+#pragma unroll
+    for (int j = 0; j < kHistBatch; ++j)
+    {
+      const uint64_t i = i0 + uint64_t(j) * kHistThreads;
+      k[j]             = i < e ? keys[i] : UK(0);
+    }
+#pragma unroll
+    for (int j = 0; j < kHistBatch; ++j)
+    {
+      if (i0 + uint64_t(j) * kHistThreads < e)
+      {
+        const UK t = tw(k[j]);
+        for (int pass = 0; pass < p.num_passes; ++pass)
+        {
+          const int shift     = p.begin_bit + pass * kRadixBits;
+          const int bits      = min(kRadixBits, p.end_bit - shift);
+          const uint32_t mask = bits > 0 ? (1u << bits) - 1u : 0u;
+          const uint32_t d    = extract_digit(t, shift, mask);
+          __hip_atomic_fetch_add(
+            &s_hist[(pass * kRadix + d) * kHistParts + part], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
     }
   }
   __syncthreads();
