@@ -177,16 +177,10 @@ def leg_wide_keys(torch, ca, dev, steps):
     return out
 
 
-def leg_segmented(torch, ca, dev, steps):
+def segmented_workload(torch, dev):
     """100,000 lognormal segments (median 512, sigma 2, clipped to [1, 2^21], rescaled to
     2^28 items), uniform uint32 keys, uint32 values = index, int32 offsets."""
-    import numpy as np
-
-    from cccl_amd import _bindings as B
-    from cccl_amd import selector as S
     from cccl_amd import workloads as W
-    from cccl_amd.graph import SortGraph
-    from cccl_amd.radix_sort import _stream_handle, key_type_of
 
     n, nseg = 1 << 28, 100000
     sizes = W.lognormal_sizes(nseg, n, 42)
@@ -194,6 +188,32 @@ def leg_segmented(torch, ca, dev, steps):
     kin = gen_keys(torch, n, "uniform", 45, dev)
     vin = torch.arange(n, dtype=torch.int32, device=dev).view(torch.uint32)
     kout, vout = torch.empty_like(kin), torch.empty_like(vin)
+    return n, nseg, sizes, begins, ends, kin, vin, kout, vout
+
+
+def segmented_profile(torch, dev, steps):
+    """The segmented batch with the library's default configuration only (for a kernel
+    trace of its launches)."""
+    from cccl_amd.radix_sort import SegmentedSorter
+
+    n, nseg, _, begins, ends, kin, vin, kout, vout = segmented_workload(torch, dev)
+    s = SegmentedSorter(torch.uint32, 4, n, nseg)
+    ms = time_steps(torch, lambda: s.sort(kin, kout, begins, ends, vin, vout), steps, 2)
+    return {"segmented_profile": "100000 lognormal segments, 2^28 u32 pairs, library default configuration",
+            "sorts": steps + 2, "ms": round(ms, 4)}
+
+
+def leg_segmented(torch, ca, dev, steps):
+    """100,000 lognormal segments (median 512, sigma 2, clipped to [1, 2^21], rescaled to
+    2^28 items), uniform uint32 keys, uint32 values = index, int32 offsets."""
+    import numpy as np
+
+    from cccl_amd import _bindings as B
+    from cccl_amd import selector as S
+    from cccl_amd.graph import SortGraph
+    from cccl_amd.radix_sort import _stream_handle, key_type_of
+
+    n, nseg, sizes, begins, ends, kin, vin, kout, vout = segmented_workload(torch, dev)
 
     def run(arm):
         s = S.make_sorter(arm, torch.uint32, 4, n, False, 0, 32, nseg)
@@ -347,6 +367,8 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="headline workload only")
     ap.add_argument("--selector-calls", type=int, default=100)
     ap.add_argument("--cpu-sample-log2", type=int, default=25)
+    ap.add_argument("--segmented-profile", action="store_true",
+                    help="only the segmented batch with the default configuration (for a kernel trace)")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="counter runs: one 2 GiB device copy (reference bytes), then --steps sorts only")
     args = ap.parse_args()
@@ -361,6 +383,9 @@ def main() -> int:
     import cccl_amd as ca
 
     dev = torch.device("cuda:0")
+    if args.segmented_profile:
+        print(json.dumps(segmented_profile(torch, dev, args.steps)), flush=True)
+        return 0
     n = 1 << args.log2n
     entropies = [e for e in args.entropies.split(",") if e]
     vin = torch.arange(n, dtype=torch.int32, device=dev).view(torch.uint32)

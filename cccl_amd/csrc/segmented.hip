@@ -345,8 +345,9 @@ hipError_t segmented_radix_sort(
     cp.counters         = d_counters;
     cp.large_range      = d_large_range;
     cp.large_first_tile = d_large_first;
-    const uint64_t grid = ceil_div<uint64_t>(uint64_t(num_segments), 256);
-    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&seg_classify_kernel), grid, 256, 0, stream, &cp, 4);
+    const uint64_t grid = ceil_div<uint64_t>(uint64_t(num_segments), kClassifyThreads);
+    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&seg_classify_kernel), grid, kClassifyThreads, 0,
+                                     stream, &cp, 4);
         e != hipSuccess)
     {
       return e;

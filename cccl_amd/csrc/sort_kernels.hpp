@@ -1471,11 +1471,21 @@ struct SegClassifyParams
 
 // One thread per segment.  List order inside a class follows the atomics and is not
 // deterministic; every segment is sorted independently, so results do not depend on it.
-// Slots are claimed once per wave and class (ballot, popcount, one atomic, lane offset
-// by mbcnt), not once per segment: 100,000 segments on a handful of counters would
-// otherwise serialise on same-address atomics.
-static __global__ void __launch_bounds__(256) seg_classify_kernel(SegClassifyParams p)
+// Slots are claimed per workgroup: each wave takes its lanes' offsets within the
+// workgroup's share of each class with one LDS atomic per class (ballot, popcount, lane
+// offset by mbcnt), then one thread per class claims the workgroup's share with one
+// global atomic.  100,000 segments on a handful of counters would otherwise serialise on
+// same-address global atomics.
+constexpr int kClassifyThreads = 1024;
+
+static __global__ void __launch_bounds__(kClassifyThreads) seg_classify_kernel(SegClassifyParams p)
 {
+  __shared__ uint32_t s_cnt[kSegCounters];  // this workgroup's count per class, large slots, large tiles
+  __shared__ uint32_t s_base[kSegCounters]; // the global start of this workgroup's share of each
+  if (threadIdx.x < uint32_t(kSegCounters))
+  {
+    s_cnt[threadIdx.x] = 0;
+  }
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t b = 0, e = 0;
   if (i < p.num_segments)
@@ -1505,8 +1515,10 @@ static __global__ void __launch_bounds__(256) seg_classify_kernel(SegClassifyPar
       }
     }
   }
+  __syncthreads(); // counters cleared
   const uint32_t lane = lane_id();
-  // one-workgroup classes
+  uint32_t off        = 0; // position within this workgroup's share of the segment's class
+  uint32_t tile_off   = 0; // large class: first tile within this workgroup's tiles
   for (int c = 0; c < p.num_classes; ++c)
   {
     const uint64_t m = __ballot(cls == c);
@@ -1518,17 +1530,14 @@ static __global__ void __launch_bounds__(256) seg_classify_kernel(SegClassifyPar
     uint32_t base         = 0;
     if (lane == leader)
     {
-      base = atomicAdd(&p.counters[c], uint32_t(__builtin_popcountll(m)));
+      base = atomicAdd(&s_cnt[c], uint32_t(__builtin_popcountll(m)));
     }
     base = __shfl(base, int(leader), 64);
     if (cls == c)
     {
-      const uint32_t s            = base + mask_prefix_count(m);
-      p.class_range[c][2 * s]     = b;
-      p.class_range[c][2 * s + 1] = e;
+      off = base + mask_prefix_count(m);
     }
   }
-  // large class: slots and a contiguous tile range per segment
   const uint64_t m = __ballot(cls == kSegLarge);
   if (m != 0)
   {
@@ -1539,18 +1548,36 @@ static __global__ void __launch_bounds__(256) seg_classify_kernel(SegClassifyPar
     uint32_t slot0 = 0, tile0 = 0;
     if (lane == leader)
     {
-      slot0 = atomicAdd(&p.counters[kSegLarge], uint32_t(__builtin_popcountll(m)));
-      tile0 = atomicAdd(&p.counters[kSegLargeTiles], all);
+      slot0 = atomicAdd(&s_cnt[kSegLarge], uint32_t(__builtin_popcountll(m)));
+      tile0 = atomicAdd(&s_cnt[kSegLargeTiles], all);
     }
     slot0 = __shfl(slot0, int(leader), 64);
     tile0 = __shfl(tile0, int(leader), 64);
     if (cls == kSegLarge)
     {
-      const uint32_t s         = slot0 + mask_prefix_count(m);
-      p.large_range[2 * s]     = b;
-      p.large_range[2 * s + 1] = e;
-      p.large_first_tile[s]    = tile0 + before;
+      off      = slot0 + mask_prefix_count(m);
+      tile_off = tile0 + before;
     }
+  }
+  __syncthreads(); // every wave's share counted
+  const uint32_t t = threadIdx.x;
+  if ((t < uint32_t(p.num_classes) || t == uint32_t(kSegLarge) || t == uint32_t(kSegLargeTiles)) && t < uint32_t(kSegCounters))
+  {
+    s_base[t] = s_cnt[t] != 0u ? atomicAdd(&p.counters[t], s_cnt[t]) : 0u;
+  }
+  __syncthreads();
+  if (cls >= 0 && cls < p.num_classes)
+  {
+    const uint32_t s              = s_base[cls] + off;
+    p.class_range[cls][2 * s]     = b;
+    p.class_range[cls][2 * s + 1] = e;
+  }
+  else if (cls == kSegLarge)
+  {
+    const uint32_t s         = s_base[kSegLarge] + off;
+    p.large_range[2 * s]     = b;
+    p.large_range[2 * s + 1] = e;
+    p.large_first_tile[s]    = s_base[kSegLargeTiles] + tile_off;
   }
 }
 
