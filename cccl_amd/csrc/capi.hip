@@ -79,7 +79,6 @@ const KernelHandle g_single_tile{"single_tile"};
 const KernelHandle g_upsweep{"upsweep"};
 const KernelHandle g_scan_bins{"scan_bins"};
 const KernelHandle g_downsweep{"downsweep"};
-const KernelHandle g_init{"init_bins_and_counters"};
 
 char* dup_cstr(const std::string& s)
 {
@@ -198,7 +197,7 @@ CUresult compile_impl(
   {
     *name_slot(&r, i) = dup_cstr("");
   }
-  *name_slot(&r, 8)  = dup_cstr("cccl_amd::init_kernel");
+  *name_slot(&r, 8)  = dup_cstr("");
   *name_slot(&r, 9)  = dup_cstr("");
   *name_slot(&r, 10) = dup_cstr("");
   *build             = r;
@@ -370,7 +369,7 @@ CCCL_C_API CUresult cccl_device_radix_sort_load(cccl_device_radix_sort_build_res
   build->upsweep_kernel                = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_upsweep));
   build->scan_bins_kernel              = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_scan_bins));
   build->downsweep_kernel              = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_downsweep));
-  build->init_bins_and_counters_kernel = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_init));
+  build->init_bins_and_counters_kernel = nullptr; // no counters to clear before the passes
   build->alt_upsweep_kernel = build->alt_downsweep_kernel = nullptr;
   build->histogram_kernel = build->exclusive_sum_kernel = nullptr;
   build->init_lookback_kernel = build->onesweep_kernel = nullptr;

@@ -353,14 +353,14 @@ cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64
     pick(256, 16, CCCL_AMD_ALGO_PASSES);
     return c;
   }
-  // Mid sizes, keys of 4 or 8 bytes: CUB's onesweep (one upfront histogram, then one
-  // kernel per pass: fewer launches than three per pass) up to 2^20 items.  Keys of 1 or
-  // 2 bytes (one or two passes) gain nothing from the upfront histogram: 512 x 8 passes.
-  if (key_bytes >= 4 && n >= pow2(14) && n < pow2(20) && pick(512, 8, CCCL_AMD_ALGO_ONESWEEP))
+  // Mid sizes: 8-byte keys (eight passes) below 2^19 items run CUB's onesweep (one
+  // upfront histogram, then one kernel per pass instead of three); otherwise 256 x 8
+  // passes below 2^20 items and 512 x 8 passes above.
+  if (key_bytes == 8 && n < pow2(19) && pick(512, 8, CCCL_AMD_ALGO_ONESWEEP))
   {
     return c;
   }
-  if ((n >= pow2(20) || key_bytes <= 2) && n >= pow2(13) && pick(512, 8, CCCL_AMD_ALGO_PASSES))
+  if (n >= pow2(20) && pick(512, 8, CCCL_AMD_ALGO_PASSES))
   {
     return c;
   }
@@ -601,18 +601,16 @@ hipError_t radix_sort(
   const uint64_t blocks      = ceil_div(tiles, tpb);
   const bool need_tmp        = !pr.overwrite && passes > 1;
 
-  size_t sizes[8] = {
+  size_t sizes[6] = {
     size_t(tiles) * kRadix * sizeof(uint32_t), // tile_off
     size_t(blocks) * kRadix * sizeof(uint32_t), // block_sum
     size_t(blocks) * kRadix * sizeof(uint64_t), // block_prefix
     size_t(kRadix) * sizeof(uint64_t), // totals
-    size_t(kRadix) * sizeof(uint64_t), // digit_start
-    size_t(passes) * sizeof(uint32_t), // spine completion counters
     need_tmp ? size_t(n) * kb : 0, // extra key buffer
     need_tmp && !keys_only ? size_t(n) * vb : 0, // extra value buffer
   };
-  void* parts[8] = {};
-  if (hipError_t e = alias_temporaries(d_temp_storage, temp_storage_bytes, parts, sizes, 8); e != hipSuccess)
+  void* parts[6] = {};
+  if (hipError_t e = alias_temporaries(d_temp_storage, temp_storage_bytes, parts, sizes, 6); e != hipSuccess)
   {
     return e;
   }
@@ -624,19 +622,9 @@ hipError_t radix_sort(
   uint32_t* d_block_sum    = static_cast<uint32_t*>(parts[1]);
   uint64_t* d_block_prefix = static_cast<uint64_t*>(parts[2]);
   uint64_t* d_totals       = static_cast<uint64_t*>(parts[3]);
-  uint64_t* d_digit_start  = static_cast<uint64_t*>(parts[4]);
-  uint32_t* d_done         = static_cast<uint32_t*>(parts[5]);
-  void* d_keys_tmp2        = parts[6];
-  void* d_vals_tmp2        = parts[7];
+  void* d_keys_tmp2        = parts[4];
+  void* d_vals_tmp2        = parts[5];
 
-  {
-    InitParams ip{d_done, uint64_t(passes)};
-    if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&init_kernel), 1, 256, 0, stream, &ip, 0);
-        e != hipSuccess)
-    {
-      return e;
-    }
-  }
 
   void* d_keys_tmp = keys.alternate();
   void* d_vals_tmp = vals.alternate();
@@ -662,13 +650,6 @@ hipError_t radix_sort(
     upp.float_mode      = float_mode;
     upp.shift           = bit;
     upp.num_bits        = num_bits;
-    const bool fused    = blocks <= kMaxFusedSpineBlocks;
-    if (fused)
-    {
-      upp.done         = d_done + pass;
-      upp.block_prefix = d_block_prefix;
-      upp.digit_start  = d_digit_start;
-    }
     if (hipError_t e = launch_kernel(up->fn, blocks, kUpsweepThreads, size_t(tpb) * kRadix * sizeof(uint32_t), stream,
                                      &upp, 1);
         e != hipSuccess)
@@ -676,9 +657,8 @@ hipError_t radix_sort(
       return e;
     }
 
-    if (!fused)
     {
-      SpineParams sp{d_block_sum, d_block_prefix, d_totals, d_digit_start, d_done + pass, uint32_t(blocks)};
+      SpineParams sp{d_block_sum, d_block_prefix, d_totals, uint32_t(blocks)};
       if (hipError_t e =
             launch_kernel(reinterpret_cast<const void*>(&spine_kernel), kRadix, kRadix, 0, stream, &sp, 8);
           e != hipSuccess)
@@ -701,7 +681,7 @@ hipError_t radix_sort(
     op.small_offsets   = n * uint64_t(std::max(kb, vb)) < (uint64_t(1) << 32) ? 1 : 0;
     op.tile_off        = d_tile_off;
     op.block_prefix    = d_block_prefix;
-    op.digit_start     = d_digit_start;
+    op.totals          = d_totals;
     op.tiles_per_block = uint32_t(tpb);
     op.num_blocks      = uint32_t(blocks);
     op.num_xcds        = uint32_t(device_xcd_count());

@@ -1,11 +1,10 @@
 // gfx950 kernels of the device-wide LSD radix sort (wave64).
 //
-//   init_kernel       zeroes the per-pass spine counters
-//                     (role of DeviceRadixSortInitKernel, kernel_radix_sort.cuh:476-496)
 //   upsweep_kernel    digit counts of every tile of one pass (DeviceRadixSortUpsweepKernel,
 //                     kernel_radix_sort.cuh:85-129)
-//   spine_kernel      per-digit exclusive scan of the block counts and the digit starts
-//                     (RadixSortScanBinsKernel, kernel_radix_sort.cuh:147-186)
+//   spine_kernel      per-digit exclusive scan of the block counts and the digit totals
+//                     (RadixSortScanBinsKernel, kernel_radix_sort.cuh:147-186); the
+//                     downsweep scans the totals into digit starts
 //   onesweep_kernel   one digit pass over tiles: wave-ballot rank, LDS-staged scatter.
 //                     kModePlain / kModeSegTable: a downsweep (DeviceRadixSortDownsweepKernel,
 //                     kernel_radix_sort.cuh:244-291) reading its offsets from a spine;
@@ -24,21 +23,6 @@
 #include "registry.hpp"
 
 namespace cccl_amd {
-
-struct InitParams
-{
-  uint32_t* counters;
-  uint64_t num_counters;
-};
-
-static __global__ void __launch_bounds__(256) init_kernel(InitParams p)
-{
-  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < p.num_counters; i += stride)
-  {
-    p.counters[i] = 0;
-  }
-}
 
 // The one segment [0, n) through which the plain sort's onesweep algorithm runs the
 // batched segmented pass: offsets[0] = begin, offsets[1] = end.
@@ -90,15 +74,7 @@ struct UpsweepParams
   int float_mode;
   int shift;
   int num_bits;
-  // Spine folded into the upsweep (few blocks): the workgroup that finishes last scans
-  // the block sums itself; null done = a separate spine_kernel launch does it.
-  uint32_t* done; // zero on entry
-  uint64_t* block_prefix;
-  uint64_t* digit_start;
 };
-
-// Largest block count for which the upsweep's last workgroup does the spine's work.
-constexpr uint32_t kMaxFusedSpineBlocks = 64;
 
 // Largest number of tiles one upsweep workgroup counts (its LDS holds one row of
 // counters per tile).
@@ -221,48 +197,6 @@ __device__ __forceinline__ void upsweep_block(const UpsweepParams& p, uint32_t* 
     }
     p.block_sum[uint64_t(tid) * p.num_blocks + b] = run;
   }
-  if (p.done == nullptr)
-  {
-    return;
-  }
-  // Folded spine.  The full barrier waits for every wave's block_sum stores, so thread
-  // 0's release covers them; the workgroup whose increment completes the count sees
-  // every block's sums.
-  __shared__ uint64_t s_wave[kRadix / 64];
-  __shared__ uint32_t s_last;
-  __syncthreads();
-  if (tid == 0)
-  {
-    const uint32_t prev = __hip_atomic_fetch_add(p.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last              = prev + 1u == p.num_blocks ? 1u : 0u;
-  }
-  __syncthreads();
-  if (s_last == 0u) // block-uniform
-  {
-    return;
-  }
-  // Every wave acquires at device scope (its stale cached lines are dropped), so the
-  // plain loads below see every block's sums and can all be in flight at once.
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  uint64_t total = 0;
-  if (tid < uint32_t(kRadix))
-  {
-    const uint32_t* col = p.block_sum + uint64_t(tid) * p.num_blocks;
-    uint64_t* pre       = p.block_prefix + uint64_t(tid) * p.num_blocks;
-#pragma unroll 8
-    for (uint32_t j = 0; j < p.num_blocks; ++j)
-    {
-      const uint32_t c = col[j];
-      pre[j]           = total;
-      total += c;
-    }
-  }
-  uint64_t all           = 0;
-  const uint64_t start   = radix_exclusive_scan<uint64_t>(total, s_wave, all);
-  if (tid < uint32_t(kRadix))
-  {
-    p.digit_start[tid] = start;
-  }
 }
 
 template <class UK, int TILE>
@@ -291,19 +225,16 @@ struct SpineParams
 {
   const uint32_t* block_sum; // [kRadix][num_blocks]
   uint64_t* block_prefix; // [kRadix][num_blocks]: exclusive prefix over blocks
-  uint64_t* totals; // [kRadix]
-  uint64_t* digit_start; // [kRadix]: exclusive prefix of totals over digits
-  uint32_t* done; // zero on entry
+  uint64_t* totals; // [kRadix]: count of each digit over the whole pass
   uint32_t num_blocks;
 };
 
-// One workgroup per digit scans that digit's block sums.  The workgroup that finishes
-// last (an acquire-release counter orders every other workgroup's total before it) also
-// scans the digit totals into digit starts.
+// One workgroup per digit scans that digit's block sums and stores the digit's total;
+// the downsweep turns the totals into digit starts inside its own scan over digits, so
+// no workgroup has to wait for the others here.
 static __global__ void __launch_bounds__(kRadix) spine_kernel(SpineParams p)
 {
   __shared__ uint64_t s_wave[kRadix / 64];
-  __shared__ uint32_t s_last;
   const uint32_t d     = blockIdx.x;
   const uint32_t tid   = threadIdx.x;
   const uint32_t nb    = p.num_blocks;
@@ -327,16 +258,7 @@ static __global__ void __launch_bounds__(kRadix) spine_kernel(SpineParams p)
   }
   if (tid == 0)
   {
-    p.totals[d]         = all;
-    const uint32_t prev = __hip_atomic_fetch_add(p.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last              = prev == uint32_t(kRadix - 1) ? 1u : 0u;
-  }
-  __syncthreads();
-  if (s_last != 0u) // block-uniform
-  {
-    const uint64_t v = __hip_atomic_load(&p.totals[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint64_t grand   = 0;
-    p.digit_start[tid] = radix_exclusive_scan<uint64_t>(v, s_wave, grand);
+    p.totals[d] = all;
   }
 }
 
@@ -369,7 +291,7 @@ struct OnesweepParams
   // plain only
   const uint32_t* tile_off; // [num_tiles][kRadix]
   const uint64_t* block_prefix; // [kRadix][num_blocks]
-  const uint64_t* digit_start; // [kRadix]
+  const uint64_t* totals; // [kRadix]: count of each digit over the pass
   uint32_t tiles_per_block;
   uint32_t num_blocks;
   uint32_t num_xcds; // workgroup w runs on XCD w % num_xcds
@@ -503,7 +425,8 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
 
   zero_wave_counters(wc);
   uint32_t tile  = blockIdx.x;
-  uint64_t gbase = 0; // table modes, thread d < kRadix: first output position of digit d in this tile
+  uint64_t gbase  = 0; // table modes, thread d < kRadix: first output position of digit d in this tile
+  uint64_t gtotal = 0; // plain, thread d < kRadix: count of digit d over the pass
   if constexpr (LOOKBACK)
   {
     // Tile ids follow the order in which workgroups start (an atomic counter, not
@@ -552,8 +475,9 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
       // Issued first: these loads complete while the keys are ranked.
       if (tid < uint32_t(kRadix))
       {
-        gbase = p.digit_start[tid] + p.block_prefix[uint64_t(tid) * p.num_blocks + tile / p.tiles_per_block]
-              + p.tile_off[uint64_t(tile) * kRadix + tid];
+        gbase  = p.block_prefix[uint64_t(tid) * p.num_blocks + tile / p.tiles_per_block]
+               + p.tile_off[uint64_t(tile) * kRadix + tid];
+        gtotal = p.totals[tid];
       }
     }
   }
@@ -671,9 +595,36 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
                    (tile == first_tile ? kFlagGlobal : kFlagPartial) | valid_total);
     }
   }
-  uint32_t tile_items  = 0;
-  const uint32_t start = radix_exclusive_scan<uint32_t, true>(total, s_misc + 4, tile_items);
-  uint64_t gstart      = 0; // look-back, thread d < kRadix: first output position of digit d in the segment
+  // start: digit d's first slot in this tile.  Plain mode also needs dstart, digit d's
+  // first output position over the pass (the exclusive scan of the pass totals): below
+  // 2^32 items both come out of one scan of (total << 32 | tile count), the tile counts
+  // never carrying into the high word.
+  uint32_t start  = 0;
+  uint64_t dstart = 0;
+  if constexpr (!SEG)
+  {
+    uint64_t* s_scan64 = reinterpret_cast<uint64_t*>(s_misc + 4);
+    if (p.num_items < (uint64_t(1) << 32)) // block-uniform
+    {
+      uint64_t all      = 0;
+      const uint64_t ex = radix_exclusive_scan<uint64_t, true>((gtotal << 32) | total, s_scan64, all);
+      start             = uint32_t(ex);
+      dstart            = ex >> 32;
+    }
+    else
+    {
+      uint32_t tile_items = 0;
+      start               = radix_exclusive_scan<uint32_t, true>(total, s_misc + 4, tile_items);
+      uint64_t all        = 0;
+      dstart              = radix_exclusive_scan<uint64_t, true>(gtotal, s_scan64, all);
+    }
+  }
+  else
+  {
+    uint32_t tile_items = 0;
+    start               = radix_exclusive_scan<uint32_t, true>(total, s_misc + 4, tile_items);
+  }
+  uint64_t gstart = 0; // look-back, thread d < kRadix: first output position of digit d in the segment
   if constexpr (LOOKBACK)
   {
     gstart = seg_begin + seg_start;
@@ -689,7 +640,7 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
     }
     if constexpr (!LOOKBACK)
     {
-      s_base[tid] = gbase - start;
+      s_base[tid] = gbase + dstart - start;
     }
   }
   lds_barrier();

@@ -36,9 +36,9 @@ CCCL_C_EXTERN_C_BEGIN
 /* Field order and types as c/parallel/include/cccl/c/radix_sort.h:27-64.
  * The 11 kernel handles name the reference's kernels.  Each digit pass here is an
  * upsweep, a scan of the bins and a downsweep, so load sets single_tile, upsweep,
- * scan_bins, downsweep and init_bins_and_counters; the handles of kernels this design
- * does not have (the alternates, histogram, exclusive_sum, init_lookback, onesweep) stay
- * NULL. */
+ * scan_bins and downsweep; the handles of kernels this design does not have (the
+ * alternates, histogram, exclusive_sum, init_bins_and_counters, init_lookback, onesweep)
+ * stay NULL. */
 typedef struct cccl_device_radix_sort_build_result_t
 {
   int cc;

@@ -158,7 +158,7 @@ CCCL_C_API int cccl_amd_graph_destroy(void* graph_exec);
 /* Kernel timing for benchmarks: while enabled, every kernel the sorts launch is
  * bracketed by a pair of HIP events on its own stream.  read waits for the recorded
  * events, writes up to max (kind, milliseconds) pairs and returns how many there were.
- * Kinds: 0 init, 1 upsweep, 2 digit pass (downsweep or onesweep), 3 one-workgroup sort,
+ * Kinds: 0 setup (the onesweep algorithm's segment offsets), 1 upsweep, 2 digit pass (downsweep or onesweep), 3 one-workgroup sort,
  * 4 segmented binning, 5 segmented histogram and digit starts, 6 segmented digit pass,
  * 7 copy, 8 spine (bin scan), 9 segmented upsweep, 10 segmented spine. */
 CCCL_C_API void cccl_amd_profile_enable(int on);

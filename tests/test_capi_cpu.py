@@ -59,7 +59,7 @@ def test_build_load_cleanup():
     assert rc == B.CUDA_SUCCESS
     assert br.payload and br.payload_size > 0 and br.runtime_policy and br.runtime_policy_size > 0
     assert br.downsweep_kernel and br.upsweep_kernel and br.scan_bins_kernel and br.single_tile_kernel
-    assert br.init_bins_and_counters_kernel and not br.onesweep_kernel
+    assert not br.init_bins_and_counters_kernel and not br.onesweep_kernel
     assert br.key_type.size == 4 and br.value_type.size == 4 and br.order == B.CCCL_ASCENDING
     assert b"upsweep" in br.upsweep_kernel_lowered_name and b"spine" in br.scan_bins_kernel_lowered_name
     assert L.cccl_device_radix_sort_cleanup(ctypes.byref(br)) == B.CUDA_SUCCESS
