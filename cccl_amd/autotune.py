@@ -243,35 +243,50 @@ def judge(records, choice) -> dict:
     return out
 
 
+def call_weights(records, alpha: float) -> torch.Tensor:
+    """Per-call loss weights (best time / geometric mean of the best times) ** alpha:
+    alpha 0 weighs every call alike in log time; above 0 the long calls, which decide the
+    total time and so the regret, count more."""
+    best = torch.tensor([min(t for t in r["times_ms"] if t is not None) for r in records], dtype=torch.float64)
+    w = (best / torch.exp(torch.log(best).mean())) ** alpha
+    return w.to(torch.float32)
+
+
 def train_and_judge(records, train_frac=0.7, epochs=600, seed=0, device="cuda", epoch_choices=None,
-                    val_frac=0.15):
+                    val_frac=0.15, weight_choices=(0.0,)):
     """Trains on the first train_frac of the stream and judges on the rest.  With
-    epoch_choices, the epoch count is picked on a validation slice (the last val_frac of
-    the training calls, networks trained on the calls before it), then the network is
-    trained on all training calls; the judged calls play no part in the choice."""
+    epoch_choices, the epoch count and the loss weighting (weight_choices: alphas of
+    call_weights) are picked on a validation slice (the last val_frac of the training
+    calls, networks trained on the calls before it), then the network is trained on all
+    training calls; the judged calls play no part in the choice."""
     n_train = int(len(records) * train_frac)
     X, T, M = dataset(records)
     X, T, M = X.to(device), T.to(device), M.to(device)
     tr, te = slice(0, n_train), slice(n_train, None)
     valid = M > 0
     chosen = {}
+    alpha = weight_choices[0] if weight_choices else 0.0
     if epoch_choices:
         n_fit = int(n_train * (1.0 - val_frac))
         fit, va = slice(0, n_fit), slice(n_fit, n_train)
-        for e in epoch_choices:
-            cand = S.MLPSelector(seed=seed, device=device)
-            cand.fit(X[fit], T[fit], M[fit], epochs=e, seed=seed)
-            pick_va = cand.choose(X[va], valid[va]).cpu().tolist()
-            chosen[e] = judge(records[n_fit:n_train], pick_va)["regret_vs_exhaustive_excl_features"]
-        epochs = min(chosen, key=lambda e: (chosen[e], e))
+        for a in weight_choices or (0.0,):
+            Mw = M * call_weights(records, a).to(device)[:, None]
+            for e in epoch_choices:
+                cand = S.MLPSelector(seed=seed, device=device)
+                cand.fit(X[fit], T[fit], Mw[fit], epochs=e, seed=seed)
+                pick_va = cand.choose(X[va], valid[va]).cpu().tolist()
+                chosen[f"{e}@{a}"] = judge(records[n_fit:n_train], pick_va)["regret_vs_exhaustive_excl_features"]
+        best_key = min(chosen, key=lambda k: (chosen[k], int(k.split("@")[0])))
+        epochs, alpha = int(best_key.split("@")[0]), float(best_key.split("@")[1])
+    Mw = M * call_weights(records, alpha).to(device)[:, None]
     net = S.MLPSelector(seed=seed, device=device)
     untrained = S.MLPSelector(seed=seed, device=device)
-    losses = net.fit(X[tr], T[tr], M[tr], epochs=epochs, seed=seed)
+    losses = net.fit(X[tr], T[tr], Mw[tr], epochs=epochs, seed=seed)
     pick = net.choose(X[te], valid[te]).cpu().tolist()
     pick_untrained = untrained.choose(X[te], valid[te]).cpu().tolist()
     held = records[n_train:]
-    out = {"train_calls": n_train, "test_calls": len(held), "epochs": epochs,
-           "validation_regret_by_epochs": chosen, "losses": losses,
+    out = {"train_calls": n_train, "test_calls": len(held), "epochs": epochs, "loss_weight_alpha": alpha,
+           "validation_regret_by_epochs_at_alpha": chosen, "losses": losses,
            "learned": judge(held, pick), "untrained": judge(held, pick_untrained)}
     # a hand rule a caller might write: our built-in default shapes
     return net, out
@@ -319,6 +334,8 @@ def main(argv=None) -> int:
     b.add_argument("--epochs", type=int, default=600)
     b.add_argument("--epoch-choices", default="600,1500,3000",
                    help="epoch counts to pick from on a validation slice of the training calls ('' = use --epochs)")
+    b.add_argument("--weight-choices", default="0,0.5",
+                   help="loss-weight exponents (call_weights) to pick from on the same validation slice")
     b.add_argument("--report", default=None)
     args = ap.parse_args(argv)
     if args.cmd == "stream":
@@ -349,7 +366,9 @@ def main(argv=None) -> int:
     if data["arms"] != [x.name for x in S.ARMS]:
         raise SystemExit("timings were recorded for a different arm list")
     choices = [int(e) for e in args.epoch_choices.split(",") if e.strip()]
-    net, rep = train_and_judge(data["records"], epochs=args.epochs, epoch_choices=choices or None)
+    alphas = tuple(float(a) for a in args.weight_choices.split(",") if a.strip()) or (0.0,)
+    net, rep = train_and_judge(data["records"], epochs=args.epochs, epoch_choices=choices or None,
+                               weight_choices=alphas)
     net.save(args.weights, meta={"trained_on": args.timings, "report": rep["learned"]})
     txt = json.dumps(rep, indent=1)
     if args.report:

@@ -49,29 +49,38 @@ __device__ __forceinline__ T wave_sum(T v)
 }
 
 // Digit entropies of a strided sample of S = min(n, 4096) keys (key i*n/S), one per
-// digit pass, in bits / 8 (1.0 = uniform 8-bit digits).  All sample loads of a thread
-// are issued before any is used.
+// digit pass, in bits / 8 (1.0 = uniform 8-bit digits).  The sample is loaded first
+// (load_samples) so its loads are in flight with the segment offsets' loads; then
+// sample_entropies counts it.
+constexpr int kPer = kSample / kFeatThreads;
+
 template <class UK>
-__device__ void sample_entropies(const FeatureParams& p, uint32_t* s_hist, float* s_red, float* ent, int passes)
+__device__ void load_samples(const FeatureParams& p, uint64_t (&ks)[kPer])
 {
-  constexpr int kPer = kSample / kFeatThreads;
-  const UK* keys     = static_cast<const UK*>(p.keys);
-  const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
+  const UK* keys   = static_cast<const UK*>(p.keys);
   const uint64_t n = p.num_items;
   const int S      = n < uint64_t(kSample) ? int(n) : kSample;
-  UK k[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j)
   {
     const int i = j * kFeatThreads + int(threadIdx.x);
-    k[j]        = i < S ? keys[(uint64_t(i) * n) / uint64_t(S)] : UK(0);
+    ks[j]       = i < S ? uint64_t(keys[(uint64_t(i) * n) / uint64_t(S)]) : 0u;
   }
+}
+
+template <class UK>
+__device__ void sample_entropies(const FeatureParams& p, const uint64_t (&ks)[kPer], uint32_t* s_hist, float* s_red,
+                                 float* ent, int passes)
+{
+  const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
+  const uint64_t n = p.num_items;
+  const int S      = n < uint64_t(kSample) ? int(n) : kSample;
 #pragma unroll
   for (int j = 0; j < kPer; ++j)
   {
     if (j * kFeatThreads + int(threadIdx.x) < S)
     {
-      const UK t = tw(k[j]);
+      const UK t = tw(UK(ks[j]));
       for (int q = 0; q < passes; ++q)
       {
         const int shift     = p.begin_bit + q * kRadixBits;
@@ -156,6 +165,26 @@ __global__ void __launch_bounds__(kFeatThreads) features_kernel(FeatureParams p)
   {
     s_hist[i] = 0;
   }
+  const bool sample = p.num_items > 0 && p.end_bit > p.begin_bit;
+  uint64_t ks[kPer] = {};
+  if (sample)
+  {
+    switch (p.key_bytes)
+    {
+      case 1:
+        load_samples<uint8_t>(p, ks);
+        break;
+      case 2:
+        load_samples<uint16_t>(p, ks);
+        break;
+      case 4:
+        load_samples<uint32_t>(p, ks);
+        break;
+      default:
+        load_samples<uint64_t>(p, ks);
+        break;
+    }
+  }
   uint64_t acc[4] = {0, 0, 0, 0};
   if (p.num_segments > 0)
   {
@@ -178,21 +207,21 @@ __global__ void __launch_bounds__(kFeatThreads) features_kernel(FeatureParams p)
   }
   __syncthreads(); // counters cleared
   const int passes = max(1, (p.end_bit - p.begin_bit + kRadixBits - 1) / kRadixBits);
-  if (p.num_items > 0 && p.end_bit > p.begin_bit)
+  if (sample)
   {
     switch (p.key_bytes)
     {
       case 1:
-        sample_entropies<uint8_t>(p, s_hist, s_red, s_ent, passes);
+        sample_entropies<uint8_t>(p, ks, s_hist, s_red, s_ent, passes);
         break;
       case 2:
-        sample_entropies<uint16_t>(p, s_hist, s_red, s_ent, passes);
+        sample_entropies<uint16_t>(p, ks, s_hist, s_red, s_ent, passes);
         break;
       case 4:
-        sample_entropies<uint32_t>(p, s_hist, s_red, s_ent, passes);
+        sample_entropies<uint32_t>(p, ks, s_hist, s_red, s_ent, passes);
         break;
       default:
-        sample_entropies<uint64_t>(p, s_hist, s_red, s_ent, passes);
+        sample_entropies<uint64_t>(p, ks, s_hist, s_red, s_ent, passes);
         break;
     }
   }
