@@ -292,6 +292,9 @@ struct OnesweepParams
   const uint32_t* tile_off; // [num_tiles][kRadix]
   const uint64_t* block_prefix; // [kRadix][num_blocks]
   const uint64_t* totals; // [kRadix]: count of each digit over the pass
+  uint8_t* next_digits; // non-null: each key's digit of the next pass, written at its output position
+  int next_shift;
+  int next_bits;
   uint32_t tiles_per_block;
   uint32_t num_blocks;
   uint32_t num_xcds; // workgroup w runs on XCD w % num_xcds
@@ -703,9 +706,17 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
       for (int j = 0; j < kChunk; ++j)
       {
         const uint32_t pos = uint32_t(c0 + j) * THREADS + tid;
-        const uint32_t d   = extract_digit(tw(k[j]), p.shift, dmask);
+        const UK t         = tw(k[j]);
+        const uint32_t d   = extract_digit(t, p.shift, dmask);
         at[j]              = s_base32[2 * d] + pos;
         store_at(kout, at[j], k[j]);
+        if constexpr (!SEG)
+        {
+          if (p.next_digits != nullptr) // block-uniform
+          {
+            store_at(p.next_digits, at[j], uint8_t(extract_digit(t, p.next_shift, (1u << p.next_bits) - 1u)));
+          }
+        }
         if constexpr (kTwoRound)
         {
           dig[(c0 + j) / 4] |= d << (8 * ((c0 + j) % 4));
@@ -730,9 +741,17 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
       if (pos < valid)
       {
         const UK k        = s_keys[pos];
-        const uint32_t d  = extract_digit(tw(k), p.shift, dmask);
+        const UK t        = tw(k);
+        const uint32_t d  = extract_digit(t, p.shift, dmask);
         const uint64_t at = s_base[d] + pos;
         kout[at]          = k;
+        if constexpr (!SEG)
+        {
+          if (p.next_digits != nullptr) // block-uniform
+          {
+            p.next_digits[at] = uint8_t(extract_digit(t, p.next_shift, (1u << p.next_bits) - 1u));
+          }
+        }
         if constexpr (kTwoRound)
         {
           dig[i / 4] |= d << (8 * (i % 4));
