@@ -338,6 +338,12 @@ cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64
     {
       return c;
     }
+    // 8-byte keys with 4-byte values below 2^30 items: the onesweep with look-back
+    // (1024 x 16, values staged after the keys), 3-6% ahead of the passes at 2^27.
+    if (key_bytes == 8 && val_bytes == 4 && n < pow2(30) && pick(1024, 16, CCCL_AMD_ALGO_ONESWEEP))
+    {
+      return c;
+    }
     if (val_bytes > 0 && item_bytes >= 12 && item_bytes <= 16 && pick(1024, 8, CCCL_AMD_ALGO_PASSES))
     {
       return c;
