@@ -7,8 +7,9 @@
 //                 class; large segments get a contiguous range of onesweep tiles
 //   seg_prep      per large segment: map its tiles to it; with look-back passes also
 //                 clear its digit counts
-//   seg_histogram per large tile (look-back passes): clear the tile's first-pass
-//                 look-back words; digit counts of every pass into its segment's counts
+//   seg_histogram per range of large tiles (look-back passes): clear each tile's
+//                 first-pass look-back words; digit counts of every pass into the
+//                 segments' counts
 //   seg_starts    per large segment (look-back passes): the counts become digit starts
 //                 within the segment
 //   block_sort    one launch per one-workgroup class, one workgroup per segment
@@ -388,7 +389,9 @@ hipError_t segmented_radix_sort(
     hp.begin_bit       = pr.begin_bit;
     hp.end_bit         = pr.end_bit;
     hp.num_passes      = passes;
-    if (hipError_t e = launch_kernel(tab->seg_histogram, tile_bound, kHistThreads, 0, stream, &hp, 5); e != hipSuccess)
+    // Each workgroup counts a contiguous range of tiles (the tile count is on the device).
+    const uint64_t hist_grid = std::min<uint64_t>(tile_bound, uint64_t(device_cu_count()) * 4);
+    if (hipError_t e = launch_kernel(tab->seg_histogram, hist_grid, kHistThreads, 0, stream, &hp, 5); e != hipSuccess)
     {
       return e;
     }

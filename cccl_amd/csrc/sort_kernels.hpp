@@ -1779,80 +1779,105 @@ struct SegHistogramParams
   int num_passes;
 };
 
-// One workgroup per large tile (grid = upper bound): digit counts of every pass for the
-// tile's items, added into its segment's counts.  Each thread has kHistBatch key loads in
-// flight before it counts them.
+// Digit counts of every pass over the large class's tiles, added into each segment's
+// counts.  Workgroup w takes a contiguous range of about total / gridDim.x tiles, counts
+// them in LDS and adds the counts into the segment's bins when the next tile belongs to
+// another segment and at the end of its range: one global atomic per (segment, pass,
+// digit) per workgroup instead of per tile (one segment's bins are otherwise hit by
+// every tile).  Each thread has kHistBatch key loads in flight before it counts them.
 constexpr int kHistBatch = 16;
+
 template <class UK>
 __global__ void __launch_bounds__(kHistThreads) seg_histogram_kernel(SegHistogramParams p)
 {
   constexpr int kMaxPasses = int(sizeof(UK) * 8 / kRadixBits);
-  __shared__ uint32_t s_hist[kMaxPasses * kRadix * kHistParts];
-  const uint32_t tile = blockIdx.x;
-  if (tile >= p.counters[kSegLargeTiles])
+  constexpr int kCounters  = kMaxPasses * kRadix * kHistParts;
+  __shared__ uint32_t s_hist[kCounters];
+  const uint64_t total = p.counters[kSegLargeTiles];
+  const uint64_t per   = (total + gridDim.x - 1) / gridDim.x;
+  const uint64_t t0    = uint64_t(blockIdx.x) * per;
+  const uint64_t t1    = min(t0 + per, total);
+  if (t0 >= t1)
   {
-    return;
+    return; // block-uniform: no tiles for this workgroup
   }
-  for (int i = threadIdx.x; i < kMaxPasses * kRadix * kHistParts; i += kHistThreads)
+  for (int i = threadIdx.x; i < kCounters; i += kHistThreads)
   {
     s_hist[i] = 0;
   }
   __syncthreads();
-  const uint32_t slot    = p.tile_slot[tile];
-  const uint64_t b       = uint64_t(p.slot_range[2 * slot]) + uint64_t(tile - p.slot_first_tile[slot]) * p.large_tile;
-  const uint64_t e       = min(uint64_t(p.slot_range[2 * slot + 1]), b + p.large_tile);
-  const UK* keys         = static_cast<const UK*>(p.keys);
+  const UK* keys      = static_cast<const UK*>(p.keys);
   const Twiddle<UK> tw{UK(p.xor_mask), p.float_mode};
   const uint32_t part = threadIdx.x % kHistParts;
-  if (p.lookback != nullptr)
-  {
-    for (uint32_t i = threadIdx.x; i < uint32_t(kRadix); i += kHistThreads)
+
+  // Adds the counted digits into slot's bins and clears the LDS counters.
+  auto flush = [&](uint32_t slot) {
+    __syncthreads();
+    uint32_t* bins = p.seg_bins + uint64_t(slot) * p.num_passes * kRadix;
+    for (int i = threadIdx.x; i < p.num_passes * kRadix; i += kHistThreads)
     {
-      p.lookback[uint64_t(tile) * kRadix + i] = 0u;
-    }
-  }
-  for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += uint64_t(kHistThreads) * kHistBatch)
-  {
-    UK k[kHistBatch];
+      uint32_t s = 0;
 #pragma unroll
-    for (int j = 0; j < kHistBatch; ++j)
-    {
-      const uint64_t i = i0 + uint64_t(j) * kHistThreads;
-      k[j]             = i < e ? keys[i] : UK(0);
-    }
-#pragma unroll
-    for (int j = 0; j < kHistBatch; ++j)
-    {
-      if (i0 + uint64_t(j) * kHistThreads < e)
+      for (int q = 0; q < kHistParts; ++q)
       {
-        const UK t = tw(k[j]);
-        for (int pass = 0; pass < p.num_passes; ++pass)
+        s += s_hist[i * kHistParts + q];
+        s_hist[i * kHistParts + q] = 0;
+      }
+      if (s != 0)
+      {
+        __hip_atomic_fetch_add(&bins[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __syncthreads();
+  };
+
+  uint32_t cur = p.tile_slot[t0];
+  for (uint64_t tile = t0; tile < t1; ++tile)
+  {
+    const uint32_t slot = p.tile_slot[tile]; // block-uniform
+    if (slot != cur)
+    {
+      flush(cur);
+      cur = slot;
+    }
+    if (p.lookback != nullptr)
+    {
+      for (uint32_t i = threadIdx.x; i < uint32_t(kRadix); i += kHistThreads)
+      {
+        p.lookback[tile * kRadix + i] = 0u;
+      }
+    }
+    const uint64_t b = uint64_t(p.slot_range[2 * slot]) + (tile - p.slot_first_tile[slot]) * p.large_tile;
+    const uint64_t e = min(uint64_t(p.slot_range[2 * slot + 1]), b + p.large_tile);
+    for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += uint64_t(kHistThreads) * kHistBatch)
+    {
+      UK k[kHistBatch];
+#pragma unroll
+      for (int j = 0; j < kHistBatch; ++j)
+      {
+        const uint64_t i = i0 + uint64_t(j) * kHistThreads;
+        k[j]             = i < e ? keys[i] : UK(0);
+      }
+#pragma unroll
+      for (int j = 0; j < kHistBatch; ++j)
+      {
+        if (i0 + uint64_t(j) * kHistThreads < e)
         {
-          const int shift     = p.begin_bit + pass * kRadixBits;
-          const int bits      = min(kRadixBits, p.end_bit - shift);
-          const uint32_t mask = bits > 0 ? (1u << bits) - 1u : 0u;
-          const uint32_t d    = extract_digit(t, shift, mask);
-          __hip_atomic_fetch_add(
-            &s_hist[(pass * kRadix + d) * kHistParts + part], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const UK t = tw(k[j]);
+          for (int pass = 0; pass < p.num_passes; ++pass)
+          {
+            const int shift     = p.begin_bit + pass * kRadixBits;
+            const int bits      = min(kRadixBits, p.end_bit - shift);
+            const uint32_t mask = bits > 0 ? (1u << bits) - 1u : 0u;
+            const uint32_t d    = extract_digit(t, shift, mask);
+            __hip_atomic_fetch_add(&s_hist[(pass * kRadix + d) * kHistParts + part], 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
         }
       }
     }
   }
-  __syncthreads();
-  uint32_t* bins = p.seg_bins + uint64_t(slot) * p.num_passes * kRadix;
-  for (int i = threadIdx.x; i < p.num_passes * kRadix; i += kHistThreads)
-  {
-    uint32_t s = 0;
-#pragma unroll
-    for (int q = 0; q < kHistParts; ++q)
-    {
-      s += s_hist[i * kHistParts + q];
-    }
-    if (s != 0)
-    {
-      __hip_atomic_fetch_add(&bins[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+  flush(cur);
 }
 
 } // namespace cccl_amd
