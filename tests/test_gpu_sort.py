@@ -135,6 +135,34 @@ def test_bit_windows(dtype, frac):
             assert_same(gk, gv, ek, ev)
 
 
+@pytest.mark.parametrize("dtype,window", [(np.int64, (0, 21)), (np.uint64, (3, 37)), (np.float64, (0, 64)),
+                                          (np.uint64, (60, 64))])
+@pytest.mark.parametrize("desc", [False, True])
+def test_u64_next_digit_bytes(dtype, window, desc):
+    """8-byte keys over several passes: each downsweep writes the next pass's digit bytes
+    (last passes of 5 and 2 bits included) and the next upsweep counts them."""
+    import cccl_amd as ca
+
+    rng = np.random.default_rng(81 + window[0])
+    n = 200003
+    keys = random_keys(rng, dtype, n, "and2") if dtype != np.float64 else float_keys_with_specials(rng, dtype, n)
+    vals = np.arange(n, dtype=np.uint32)
+    cfg = ca.make_config(512, 8, single_tile_threads=0)
+    tk = {np.int64: torch.int64, np.uint64: torch.uint64, np.float64: torch.float64}
+    for with_vals in (True, False):
+        s = ca.Sorter(tk[np.dtype(dtype).type], 4 if with_vals else 0, n, descending=desc, begin_bit=window[0],
+                      end_bit=window[1], config=cfg)
+        kd = to_dev(keys)
+        ko = torch.empty_like(kd)
+        vd = to_dev(vals) if with_vals else None
+        vo = torch.empty_like(vd) if with_vals else None
+        s.sort(kd, ko, vd, vo)
+        torch.cuda.synchronize()
+        ek, ev = oracle.sort(keys, vals if with_vals else None, descending=desc, begin_bit=window[0],
+                             end_bit=window[1])
+        assert_same(to_host(ko, keys.dtype), to_host(vo, np.uint32) if with_vals else None, ek, ev)
+
+
 def test_u64_window_0_40_descending():
     rng = np.random.default_rng(40)
     n = 1 << 20
