@@ -1,7 +1,8 @@
 """MI355X-native (gfx950) device-wide LSD radix sort behind CUB's / CCCL's radix-sort API.
 
 The native library libcccl_amd.so holds the kernels, the host dispatch and the
-cccl_device_radix_sort_* C-ABI; this package is its Python host binding.
+cccl_device_radix_sort_* / cccl_device_segmented_sort_* C-ABIs; this package is its
+Python host binding.
 """
 
 from .radix_sort import (  # noqa: F401
@@ -24,3 +25,4 @@ from .radix_sort import (  # noqa: F401
     radix_sort,
     segmented_radix_sort,
 )
+from .segmented_sort import make_segmented_sort, segmented_sort  # noqa: F401,E402

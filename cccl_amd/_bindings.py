@@ -1,5 +1,6 @@
-"""ctypes view of libcccl_amd.so: the cccl_device_radix_sort_* C-ABI
-(include/cccl/c/radix_sort.h) and the typed cccl_amd_* entry points
+"""ctypes view of libcccl_amd.so: the cccl_device_radix_sort_* and
+cccl_device_segmented_sort_* C-ABIs (include/cccl/c/radix_sort.h, segmented_sort.h) and
+the typed cccl_amd_* entry points
 (include/cccl_amd/radix_sort_amd.h).
 
 Struct layouts follow include/cccl/c/types.h field for field.  Loading fails loudly
@@ -98,6 +99,47 @@ class cccl_device_radix_sort_build_result_t(ctypes.Structure):
     )
 
 
+_SEG_NAMES = [
+    "segmented_sort_fallback_kernel", "segmented_sort_kernel_small", "segmented_sort_kernel_large",
+    "three_way_partition_init_kernel", "three_way_partition_kernel",
+]
+
+
+class cccl_device_segmented_sort_build_result_t(ctypes.Structure):
+    """Field order of include/cccl/c/segmented_sort.h (c/parallel/include/cccl/c/segmented_sort.h:27-53)."""
+    _fields_ = (
+        [
+            ("cc", ctypes.c_int),
+            ("payload", ctypes.c_void_p),
+            ("payload_size", ctypes.c_size_t),
+            ("payload_kind", ctypes.c_int),
+            ("library", ctypes.c_void_p),
+            ("key_type", cccl_type_info),
+            ("offset_type", cccl_type_info),
+            ("large_segments_selector_op", cccl_op_t),
+            ("small_segments_selector_op", cccl_op_t),
+        ]
+        + [(n, ctypes.c_void_p) for n in _SEG_NAMES]
+        + [
+            ("runtime_policy", ctypes.c_void_p),
+            ("runtime_policy_size", ctypes.c_size_t),
+            ("partition_runtime_policy", ctypes.c_void_p),
+            ("partition_runtime_policy_size", ctypes.c_size_t),
+            ("order", ctypes.c_int),
+        ]
+        + [(n + "_lowered_name", ctypes.c_char_p) for n in _SEG_NAMES]
+    )
+
+
+class segment_partition_policy(ctypes.Structure):
+    """cccl_amd_segment_partition_policy_t"""
+    _fields_ = [
+        ("num_classes", ctypes.c_int32),
+        ("class_max_items", ctypes.c_uint32 * 6),
+        ("large_tile_items", ctypes.c_uint32),
+    ]
+
+
 class radix_sort_config(ctypes.Structure):
     """cccl_amd_radix_sort_config_t"""
     _fields_ = [
@@ -141,6 +183,15 @@ EXPORTED_SYMBOLS = [
     "cccl_device_radix_sort_deserialize",
     "cccl_device_radix_sort_cleanup",
     "cccl_serialization_buffer_free",
+    "cccl_device_segmented_sort_build",
+    "cccl_device_segmented_sort_build_ex",
+    "cccl_device_segmented_sort_compile",
+    "cccl_device_segmented_sort_load",
+    "cccl_device_segmented_sort",
+    "cccl_device_segmented_sort_link_ltoir",
+    "cccl_device_segmented_sort_serialize",
+    "cccl_device_segmented_sort_deserialize",
+    "cccl_device_segmented_sort_cleanup",
     "cccl_amd_radix_sort",
     "cccl_amd_segmented_radix_sort",
     "cccl_amd_radix_sort_default_config",
@@ -195,6 +246,25 @@ def lib():
                  "cccl_device_radix_sort_load", "cccl_device_radix_sort", "cccl_device_radix_sort_link_ltoir",
                  "cccl_device_radix_sort_serialize", "cccl_device_radix_sort_deserialize",
                  "cccl_device_radix_sort_cleanup"):
+        getattr(L, name).restype = ctypes.c_int
+    SB = cccl_device_segmented_sort_build_result_t
+    it = cccl_iterator_t
+    seg_build_args = [ctypes.POINTER(SB), i32, it, it, it, it, i32, i32, ctypes.c_char_p, ctypes.c_char_p,
+                      ctypes.c_char_p, ctypes.c_char_p]
+    L.cccl_device_segmented_sort_build.argtypes = seg_build_args
+    L.cccl_device_segmented_sort_build_ex.argtypes = seg_build_args + [ctypes.POINTER(cccl_build_config)]
+    L.cccl_device_segmented_sort_compile.argtypes = seg_build_args + [ctypes.POINTER(cccl_build_config)]
+    L.cccl_device_segmented_sort_load.argtypes = [ctypes.POINTER(SB)]
+    L.cccl_device_segmented_sort.argtypes = [SB, vp, ctypes.POINTER(sz), it, it, it, it, u64, u64, it, it,
+                                             ctypes.c_bool, ctypes.POINTER(i32), vp]
+    L.cccl_device_segmented_sort_link_ltoir.argtypes = [ctypes.POINTER(SB), vp, vp, sz]
+    L.cccl_device_segmented_sort_serialize.argtypes = [ctypes.POINTER(SB), ctypes.POINTER(vp), ctypes.POINTER(sz)]
+    L.cccl_device_segmented_sort_deserialize.argtypes = [ctypes.POINTER(SB), vp, sz]
+    L.cccl_device_segmented_sort_cleanup.argtypes = [ctypes.POINTER(SB)]
+    for name in ("cccl_device_segmented_sort_build", "cccl_device_segmented_sort_build_ex",
+                 "cccl_device_segmented_sort_compile", "cccl_device_segmented_sort_load", "cccl_device_segmented_sort",
+                 "cccl_device_segmented_sort_link_ltoir", "cccl_device_segmented_sort_serialize",
+                 "cccl_device_segmented_sort_deserialize", "cccl_device_segmented_sort_cleanup"):
         getattr(L, name).restype = ctypes.c_int
     L.cccl_serialization_buffer_free.argtypes = [vp]
     L.cccl_serialization_buffer_free.restype = None

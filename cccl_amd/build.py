@@ -27,6 +27,7 @@ SOURCES = [
     "dispatch.hip",
     "segmented.hip",
     "capi.hip",
+    "segmented_sort_capi.hip",
     "graph.hip",
     "mlp.hip",
     "features.hip",

@@ -16,39 +16,17 @@
 #include "../../include/cccl_amd/radix_sort_amd.h"
 #include "dispatch.hpp"
 #include "registry.hpp"
+#include "capi_util.hpp"
 
 namespace {
 
+using namespace cccl_amd::capi;
 using cccl_amd::DoubleBufferRef;
 using cccl_amd::SortProblem;
 
 bool key_traits(cccl_type_enum t, int& bytes, int& kind)
 {
   return cccl_amd::key_traits_from_enum(int(t), bytes, kind);
-}
-
-bool value_size_ok(size_t v)
-{
-  return v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16;
-}
-
-const char* width_name(int bytes)
-{
-  switch (bytes)
-  {
-    case 0:
-      return "none";
-    case 1:
-      return "u8";
-    case 2:
-      return "u16";
-    case 4:
-      return "u32";
-    case 8:
-      return "u64";
-    default:
-      return "u128";
-  }
 }
 
 // Build payload: which precompiled code objects a build result selects.
@@ -79,13 +57,6 @@ const KernelHandle g_single_tile{"single_tile"};
 const KernelHandle g_upsweep{"upsweep"};
 const KernelHandle g_scan_bins{"scan_bins"};
 const KernelHandle g_downsweep{"downsweep"};
-
-char* dup_cstr(const std::string& s)
-{
-  char* p = new char[s.size() + 1];
-  std::memcpy(p, s.c_str(), s.size() + 1);
-  return p;
-}
 
 constexpr int kNumNames = 11;
 
@@ -216,106 +187,6 @@ bool read_payload(const cccl_device_radix_sort_build_result_t& b, BuildPayload& 
 
 // ---- serialization ----------------------------------------------------------
 constexpr char kBlobMagic[8] = {'C', 'C', 'A', 'M', 'D', 'R', 'S', '1'};
-
-struct Writer
-{
-  std::vector<unsigned char> buf;
-  void bytes(const void* p, size_t n)
-  {
-    const auto* c = static_cast<const unsigned char*>(p);
-    buf.insert(buf.end(), c, c + n);
-  }
-  template <class T>
-  void pod(const T& v)
-  {
-    bytes(&v, sizeof(T));
-  }
-  void blob(const void* p, size_t n)
-  {
-    pod<uint64_t>(n);
-    bytes(p, n);
-  }
-  void cstr(const char* s)
-  {
-    if (s == nullptr)
-    {
-      pod<uint32_t>(0xffffffffu);
-      return;
-    }
-    const uint32_t n = uint32_t(std::strlen(s));
-    pod<uint32_t>(n);
-    bytes(s, n);
-  }
-  void type_info(const cccl_type_info& t)
-  {
-    pod<uint64_t>(t.size);
-    pod<uint64_t>(t.alignment);
-    pod<uint32_t>(uint32_t(t.type));
-  }
-};
-
-struct Reader
-{
-  const unsigned char* p;
-  size_t left;
-  bool ok = true;
-  void bytes(void* dst, size_t n)
-  {
-    if (!ok || n > left)
-    {
-      ok = false;
-      return;
-    }
-    std::memcpy(dst, p, n);
-    p += n;
-    left -= n;
-  }
-  template <class T>
-  T pod()
-  {
-    T v{};
-    bytes(&v, sizeof(T));
-    return v;
-  }
-  char* blob_new(size_t& n)
-  {
-    const uint64_t len = pod<uint64_t>();
-    if (!ok || len > left)
-    {
-      ok = false;
-      return nullptr;
-    }
-    n       = size_t(len);
-    char* d = new char[n > 0 ? n : 1];
-    bytes(d, n);
-    return d;
-  }
-  char* cstr_new()
-  {
-    const uint32_t len = pod<uint32_t>();
-    if (!ok || len == 0xffffffffu)
-    {
-      return nullptr;
-    }
-    if (len > left)
-    {
-      ok = false;
-      return nullptr;
-    }
-    char* d = new char[size_t(len) + 1];
-    bytes(d, len);
-    d[len] = '\0';
-    return d;
-  }
-  cccl_type_info type_info()
-  {
-    cccl_type_info t{};
-    t.size      = size_t(pod<uint64_t>());
-    t.alignment = size_t(pod<uint64_t>());
-    t.type      = cccl_type_enum(pod<uint32_t>());
-    return t;
-  }
-};
 
 } // namespace
 

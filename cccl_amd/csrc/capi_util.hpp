@@ -1,0 +1,146 @@
+// Helpers shared by the C-ABI translation units: value-width checks, kernel-name
+// strings, and the byte writer / reader of the build-result serialization format.
+#pragma once
+
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <cccl/c/types.h>
+
+namespace cccl_amd::capi {
+
+inline bool value_size_ok(size_t v)
+{
+  return v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16;
+}
+
+inline const char* width_name(int bytes)
+{
+  switch (bytes)
+  {
+    case 0:
+      return "none";
+    case 1:
+      return "u8";
+    case 2:
+      return "u16";
+    case 4:
+      return "u32";
+    case 8:
+      return "u64";
+    default:
+      return "u128";
+  }
+}
+
+inline char* dup_cstr(const std::string& s)
+{
+  char* p = new char[s.size() + 1];
+  std::memcpy(p, s.c_str(), s.size() + 1);
+  return p;
+}
+
+struct Writer
+{
+  std::vector<unsigned char> buf;
+  void bytes(const void* p, size_t n)
+  {
+    const auto* c = static_cast<const unsigned char*>(p);
+    buf.insert(buf.end(), c, c + n);
+  }
+  template <class T>
+  void pod(const T& v)
+  {
+    bytes(&v, sizeof(T));
+  }
+  void blob(const void* p, size_t n)
+  {
+    pod<uint64_t>(n);
+    bytes(p, n);
+  }
+  void cstr(const char* s)
+  {
+    if (s == nullptr)
+    {
+      pod<uint32_t>(0xffffffffu);
+      return;
+    }
+    const uint32_t n = uint32_t(std::strlen(s));
+    pod<uint32_t>(n);
+    bytes(s, n);
+  }
+  void type_info(const cccl_type_info& t)
+  {
+    pod<uint64_t>(t.size);
+    pod<uint64_t>(t.alignment);
+    pod<uint32_t>(uint32_t(t.type));
+  }
+};
+
+struct Reader
+{
+  const unsigned char* p;
+  size_t left;
+  bool ok = true;
+  void bytes(void* dst, size_t n)
+  {
+    if (!ok || n > left)
+    {
+      ok = false;
+      return;
+    }
+    std::memcpy(dst, p, n);
+    p += n;
+    left -= n;
+  }
+  template <class T>
+  T pod()
+  {
+    T v{};
+    bytes(&v, sizeof(T));
+    return v;
+  }
+  char* blob_new(size_t& n)
+  {
+    const uint64_t len = pod<uint64_t>();
+    if (!ok || len > left)
+    {
+      ok = false;
+      return nullptr;
+    }
+    n       = size_t(len);
+    char* d = new char[n > 0 ? n : 1];
+    bytes(d, n);
+    return d;
+  }
+  char* cstr_new()
+  {
+    const uint32_t len = pod<uint32_t>();
+    if (!ok || len == 0xffffffffu)
+    {
+      return nullptr;
+    }
+    if (len > left)
+    {
+      ok = false;
+      return nullptr;
+    }
+    char* d = new char[size_t(len) + 1];
+    bytes(d, len);
+    d[len] = '\0';
+    return d;
+  }
+  cccl_type_info type_info()
+  {
+    cccl_type_info t{};
+    t.size      = size_t(pod<uint64_t>());
+    t.alignment = size_t(pod<uint64_t>());
+    t.type      = cccl_type_enum(pod<uint32_t>());
+    return t;
+  }
+};
+
+} // namespace cccl_amd::capi

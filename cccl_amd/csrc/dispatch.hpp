@@ -69,6 +69,21 @@ hipError_t segmented_radix_sort(
   const cccl_amd_segmented_config_t& cfg,
   hipStream_t stream);
 
+// cub::DeviceSegmentedSort over the full key: the segmented sort above with CUB's
+// DeviceSegmentedSort buffer and selector rules (bit window taken from the key width).
+hipError_t segmented_sort(
+  void* d_temp_storage,
+  size_t& temp_storage_bytes,
+  DoubleBufferRef& keys,
+  DoubleBufferRef& vals,
+  const SortProblem& prob,
+  int64_t num_segments,
+  const void* d_begin_offsets,
+  const void* d_end_offsets,
+  int offset_bytes,
+  const cccl_amd_segmented_config_t& cfg,
+  hipStream_t stream);
+
 cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64_t num_items);
 
 bool config_is_valid(const cccl_amd_radix_sort_config_t& cfg, int key_bytes, int val_bytes);

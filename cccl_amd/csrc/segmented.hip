@@ -547,6 +547,95 @@ hipError_t segmented_radix_sort(
   return hipSuccess;
 }
 
+// cub::DeviceSegmentedSort: every segment sorted whole by key (`<`, or `>` descending,
+// equal keys in input order), with the buffer and selector rules of
+// cub/cub/device/dispatch/dispatch_segmented_sort.cuh:1293-1299 and :1537-1538.  Without
+// is_overwrite_okay the result is in the alternate buffer.  With it, the result is in
+// buf[(selector + P) & 1], where P is the pass count of CUB's own large-segment radix
+// sort: ceil(8K / 6) for keys wider than one byte and ceil(8 / 4) = 2 for one-byte keys
+// (tuning_segmented_sort.cuh:193, :207, :221).  The passes here are 8-bit, so where
+// ceil(8K / 8) has the other parity the call runs as follows instead:
+//   * result due in the alternate buffer (2-, 8-byte keys): the sort without overwrite,
+//     which uses one extra key + value buffer in the temp storage and no extra traffic;
+//   * result due in the current buffer (1-byte keys): the current buffer is copied to
+//     the alternate one (whose contents are scratch under is_overwrite_okay) and sorted
+//     back from there, so items outside every segment keep their values in the current
+//     buffer.
+hipError_t segmented_sort(
+  void* d_temp_storage,
+  size_t& temp_storage_bytes,
+  DoubleBufferRef& keys,
+  DoubleBufferRef& vals,
+  const SortProblem& prob,
+  int64_t num_segments,
+  const void* d_begin_offsets,
+  const void* d_end_offsets,
+  int offset_bytes,
+  const cccl_amd_segmented_config_t& cfg,
+  hipStream_t stream)
+{
+  SortProblem pr = prob;
+  pr.begin_bit   = 0;
+  pr.end_bit     = pr.key_bytes * 8;
+  if (pr.key_bytes < 1 || pr.key_bytes > 8)
+  {
+    return hipErrorInvalidValue;
+  }
+  const int cub_passes  = ceil_div(pr.end_bit, pr.key_bytes > 1 ? 6 : 4);
+  const int own_passes  = ceil_div(pr.end_bit, kRadixBits);
+  const bool same_final = ((cub_passes ^ own_passes) & 1) == 0;
+  if (!pr.overwrite || same_final)
+  {
+    return segmented_radix_sort(
+      d_temp_storage, temp_storage_bytes, keys, vals, pr, num_segments, d_begin_offsets, d_end_offsets, offset_bytes,
+      cfg, stream);
+  }
+  const bool empty      = pr.num_items == 0 || num_segments == 0;
+  const int final_sel   = (keys.selector + cub_passes) & 1;
+  const bool to_current = final_sel == keys.selector;
+  const bool keys_only  = pr.val_bytes == 0;
+  hipError_t e          = hipSuccess;
+  if (!to_current)
+  {
+    SortProblem q = pr;
+    q.overwrite   = false;
+    DoubleBufferRef k2{{keys.current(), keys.alternate()}, 0};
+    DoubleBufferRef v2{{vals.current(), vals.alternate()}, 0};
+    e = segmented_radix_sort(
+      d_temp_storage, temp_storage_bytes, k2, v2, q, num_segments, d_begin_offsets, d_end_offsets, offset_bytes, cfg,
+      stream);
+  }
+  else
+  {
+    DoubleBufferRef k2{{keys.alternate(), keys.current()}, 0};
+    DoubleBufferRef v2{{vals.alternate(), vals.current()}, 0};
+    if (d_temp_storage != nullptr && !empty)
+    {
+      e = hipMemcpyAsync(keys.alternate(), keys.current(), size_t(pr.num_items) * pr.key_bytes,
+                         hipMemcpyDeviceToDevice, stream);
+      if (e == hipSuccess && !keys_only)
+      {
+        e = hipMemcpyAsync(vals.alternate(), vals.current(), size_t(pr.num_items) * pr.val_bytes,
+                           hipMemcpyDeviceToDevice, stream);
+      }
+      if (e != hipSuccess)
+      {
+        set_last_error("segmented sort: staging copy failed: %s", hipGetErrorString(e));
+        return e;
+      }
+    }
+    e = segmented_radix_sort(
+      d_temp_storage, temp_storage_bytes, k2, v2, pr, num_segments, d_begin_offsets, d_end_offsets, offset_bytes, cfg,
+      stream);
+  }
+  if (e == hipSuccess && d_temp_storage != nullptr && !empty)
+  {
+    keys.selector = final_sel;
+    vals.selector = final_sel;
+  }
+  return e;
+}
+
 } // namespace cccl_amd
 
 extern "C" CCCL_C_API void cccl_amd_segmented_default_config(int key_bytes, int value_bytes, cccl_amd_segmented_config_t* cfg)

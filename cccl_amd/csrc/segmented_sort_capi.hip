@@ -1,0 +1,577 @@
+// The cccl_device_segmented_sort_* C-ABI (c/parallel/include/cccl/c/segmented_sort.h)
+// on top of the batched segmented radix sort (cccl_amd::segmented_sort).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include <hip/hip_runtime.h>
+// The HIP runtime header maps CUDA_SUCCESS to hipSuccess; this file uses the C-ABI's
+// own CUresult enumerator of the same value instead.
+#undef CUDA_SUCCESS
+#define CCCL_C_EXPERIMENTAL
+#include "../../include/cccl/c/segmented_sort.h"
+#include "../../include/cccl_amd/radix_sort_amd.h"
+#include "capi_util.hpp"
+#include "dispatch.hpp"
+#include "registry.hpp"
+
+namespace {
+
+using namespace cccl_amd::capi;
+using cccl_amd::DoubleBufferRef;
+using cccl_amd::SortProblem;
+using BuildResult = cccl_device_segmented_sort_build_result_t;
+
+// Build payload: which precompiled code objects a build result selects.
+struct SegSortPayload
+{
+  char magic[8];
+  uint32_t version;
+  int32_t key_bytes;
+  int32_t key_kind;
+  int32_t value_bytes;
+  int32_t offset_bytes;
+  int32_t offset_unsigned;
+  int32_t order;
+  char arch[20];
+};
+constexpr char kPayloadMagic[8] = {'C', 'C', 'A', 'M', 'D', 'S', 'P', '1'};
+constexpr char kBlobMagic[8]    = {'C', 'C', 'A', 'M', 'D', 'S', 'S', '1'};
+
+// Handles returned by _load: CUlibrary points at the library descriptor, each CUkernel
+// at one of the kernel descriptors below.
+struct LibraryHandle
+{
+  const char* arch;
+};
+const LibraryHandle g_library{"gfx950"};
+struct KernelHandle
+{
+  const char* role;
+};
+const KernelHandle g_fallback{"seg_pass"};
+const KernelHandle g_small{"segment_sort"};
+const KernelHandle g_large{"seg_digit_pass"};
+const KernelHandle g_partition_init{"seg_init"};
+const KernelHandle g_partition{"seg_classify"};
+
+constexpr int kNumNames = 5;
+
+char** name_slot(BuildResult* b, int i)
+{
+  char** slots[kNumNames] = {
+    &b->segmented_sort_fallback_kernel_lowered_name,
+    &b->segmented_sort_kernel_small_lowered_name,
+    &b->segmented_sort_kernel_large_lowered_name,
+    &b->three_way_partition_init_kernel_lowered_name,
+    &b->three_way_partition_kernel_lowered_name,
+  };
+  return slots[i];
+}
+
+void free_build(BuildResult* b)
+{
+  delete[] static_cast<char*>(b->payload);
+  std::free(b->runtime_policy);
+  std::free(b->partition_runtime_policy);
+  for (int i = 0; i < kNumNames; ++i)
+  {
+    delete[] *name_slot(b, i);
+  }
+  std::memset(b, 0, sizeof(*b));
+}
+
+// Offset element types the segment kernels read: 32- or 64-bit integers.
+bool offset_traits(const cccl_type_info& t, int& bytes, int& is_unsigned)
+{
+  switch (t.type)
+  {
+    case CCCL_INT32:
+    case CCCL_UINT32:
+      bytes = 4;
+      break;
+    case CCCL_INT64:
+    case CCCL_UINT64:
+      bytes = 8;
+      break;
+    default:
+      return false;
+  }
+  is_unsigned = t.type == CCCL_UINT32 || t.type == CCCL_UINT64;
+  return t.size == size_t(bytes);
+}
+
+cccl_amd_segment_partition_policy_t partition_policy_of(const cccl_amd_segmented_config_t& cfg)
+{
+  cccl_amd_segment_partition_policy_t pp{};
+  pp.num_classes = cfg.num_block_classes;
+  for (int c = 0; c < cfg.num_block_classes && c < 6; ++c)
+  {
+    pp.class_max_items[c] = uint32_t(cfg.block_threads[c]) * uint32_t(cfg.block_ipt[c]);
+  }
+  pp.large_tile_items = uint32_t(cfg.large_threads) * uint32_t(cfg.large_ipt);
+  return pp;
+}
+
+CUresult compile_impl(
+  BuildResult* build,
+  cccl_sort_order_t sort_order,
+  const cccl_iterator_t& keys_it,
+  const cccl_iterator_t& values_it,
+  const cccl_iterator_t& begin_it,
+  const cccl_iterator_t& end_it,
+  int cc_major,
+  int cc_minor)
+{
+  if (build == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  if (sort_order != CCCL_ASCENDING && sort_order != CCCL_DESCENDING)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  int key_bytes = 0, key_kind = 0;
+  if (!cccl_amd::key_traits_from_enum(int(keys_it.value_type.type), key_bytes, key_kind)
+      || keys_it.value_type.size != size_t(key_bytes))
+  {
+    return CUDA_ERROR_NOT_SUPPORTED;
+  }
+  const bool keys_only = values_it.type == CCCL_POINTER && values_it.state == nullptr;
+  const size_t vbytes  = keys_only ? 0 : values_it.value_type.size;
+  if (!value_size_ok(vbytes))
+  {
+    return CUDA_ERROR_NOT_SUPPORTED;
+  }
+  int off_bytes = 0, off_unsigned = 0, end_bytes = 0, end_unsigned = 0;
+  if (begin_it.type != CCCL_POINTER || end_it.type != CCCL_POINTER
+      || !offset_traits(begin_it.value_type, off_bytes, off_unsigned)
+      || !offset_traits(end_it.value_type, end_bytes, end_unsigned) || end_bytes != off_bytes)
+  {
+    return CUDA_ERROR_NOT_SUPPORTED; // device-side offset iterators need LTO-IR linking
+  }
+  if (!((cc_major == 0 && cc_minor == 0) || (cc_major == 9 && cc_minor == 5)))
+  {
+    return CUDA_ERROR_NOT_SUPPORTED; // only gfx950 code objects are built
+  }
+
+  auto* payload = new (std::nothrow) char[sizeof(SegSortPayload)];
+  auto* policy  = static_cast<cccl_amd_segmented_config_t*>(std::malloc(sizeof(cccl_amd_segmented_config_t)));
+  auto* ppolicy =
+    static_cast<cccl_amd_segment_partition_policy_t*>(std::malloc(sizeof(cccl_amd_segment_partition_policy_t)));
+  if (payload == nullptr || policy == nullptr || ppolicy == nullptr)
+  {
+    delete[] payload;
+    std::free(policy);
+    std::free(ppolicy);
+    return CUDA_ERROR_OUT_OF_MEMORY;
+  }
+  SegSortPayload sp{};
+  std::memcpy(sp.magic, kPayloadMagic, 8);
+  sp.version         = 1;
+  sp.key_bytes       = key_bytes;
+  sp.key_kind        = key_kind;
+  sp.value_bytes     = int32_t(vbytes);
+  sp.offset_bytes    = off_bytes;
+  sp.offset_unsigned = off_unsigned && end_unsigned;
+  sp.order           = int32_t(sort_order);
+  std::snprintf(sp.arch, sizeof(sp.arch), "gfx950");
+  std::memcpy(payload, &sp, sizeof(sp));
+  cccl_amd_segmented_default_config(key_bytes, int(vbytes), policy);
+  *ppolicy = partition_policy_of(*policy);
+
+  BuildResult r{};
+  r.cc                            = 95;
+  r.payload                       = payload;
+  r.payload_size                  = sizeof(SegSortPayload);
+  r.payload_kind                  = CCCL_PAYLOAD_CUBIN;
+  r.key_type                      = keys_it.value_type;
+  r.offset_type                   = begin_it.value_type;
+  r.runtime_policy                = policy;
+  r.runtime_policy_size           = sizeof(cccl_amd_segmented_config_t);
+  r.partition_runtime_policy      = ppolicy;
+  r.partition_runtime_policy_size = sizeof(cccl_amd_segment_partition_policy_t);
+  r.order                         = sort_order;
+  const std::string kv = std::string(width_name(key_bytes)) + "," + width_name(int(vbytes));
+  *name_slot(&r, 0)    = dup_cstr("cccl_amd::seg_pass_kernel<" + kv + ">");
+  *name_slot(&r, 1)    = dup_cstr("cccl_amd::block_sort_kernel<" + kv + ">");
+  *name_slot(&r, 2)    = dup_cstr("cccl_amd::onesweep_kernel<" + kv + ",seg_table>");
+  *name_slot(&r, 3)    = dup_cstr("cccl_amd::seg_init_kernel");
+  *name_slot(&r, 4)    = dup_cstr("cccl_amd::seg_classify_kernel");
+  *build               = r;
+  return CUDA_SUCCESS;
+}
+
+bool read_payload(const BuildResult& b, SegSortPayload& out)
+{
+  if (b.payload == nullptr || b.payload_size != sizeof(SegSortPayload))
+  {
+    return false;
+  }
+  std::memcpy(&out, b.payload, sizeof(out));
+  return std::memcmp(out.magic, kPayloadMagic, 8) == 0 && out.version == 1;
+}
+
+bool policies_ok(const BuildResult& b)
+{
+  return b.runtime_policy != nullptr && b.runtime_policy_size == sizeof(cccl_amd_segmented_config_t)
+      && b.partition_runtime_policy != nullptr
+      && b.partition_runtime_policy_size == sizeof(cccl_amd_segment_partition_policy_t);
+}
+
+} // namespace
+
+extern "C" {
+
+CCCL_C_API CUresult cccl_device_segmented_sort_compile(
+  BuildResult* build,
+  cccl_sort_order_t sort_order,
+  cccl_iterator_t d_keys_in,
+  cccl_iterator_t d_values_in,
+  cccl_iterator_t begin_offset_in,
+  cccl_iterator_t end_offset_in,
+  int cc_major,
+  int cc_minor,
+  const char* /*cub_path*/,
+  const char* /*thrust_path*/,
+  const char* /*libcudacxx_path*/,
+  const char* /*ctk_path*/,
+  cccl_build_config* /*config*/)
+{
+  try
+  {
+    return compile_impl(build, sort_order, d_keys_in, d_values_in, begin_offset_in, end_offset_in, cc_major, cc_minor);
+  }
+  catch (const std::exception& exc)
+  {
+    std::printf("\nEXCEPTION in cccl_device_segmented_sort_compile(): %s\n", exc.what());
+    std::fflush(stdout);
+    return CUDA_ERROR_UNKNOWN;
+  }
+}
+
+CCCL_C_API CUresult cccl_device_segmented_sort_load(BuildResult* build)
+{
+  if (build == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  SegSortPayload sp{};
+  if (!read_payload(*build, sp) || !policies_ok(*build))
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  if (cccl_amd::kernels_for_key_bytes(sp.key_bytes) == nullptr || !value_size_ok(size_t(sp.value_bytes))
+      || (sp.offset_bytes != 4 && sp.offset_bytes != 8))
+  {
+    return CUDA_ERROR_INVALID_IMAGE;
+  }
+  build->library = reinterpret_cast<CUlibrary>(const_cast<LibraryHandle*>(&g_library));
+  build->segmented_sort_fallback_kernel  = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_fallback));
+  build->segmented_sort_kernel_small     = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_small));
+  build->segmented_sort_kernel_large     = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_large));
+  build->three_way_partition_init_kernel = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_partition_init));
+  build->three_way_partition_kernel      = reinterpret_cast<CUkernel>(const_cast<KernelHandle*>(&g_partition));
+  return CUDA_SUCCESS;
+}
+
+CCCL_C_API CUresult cccl_device_segmented_sort_build_ex(
+  BuildResult* build,
+  cccl_sort_order_t sort_order,
+  cccl_iterator_t d_keys_in,
+  cccl_iterator_t d_values_in,
+  cccl_iterator_t begin_offset_in,
+  cccl_iterator_t end_offset_in,
+  int cc_major,
+  int cc_minor,
+  const char* cub_path,
+  const char* thrust_path,
+  const char* libcudacxx_path,
+  const char* ctk_path,
+  cccl_build_config* config)
+{
+  CUresult r = cccl_device_segmented_sort_compile(
+    build,
+    sort_order,
+    d_keys_in,
+    d_values_in,
+    begin_offset_in,
+    end_offset_in,
+    cc_major,
+    cc_minor,
+    cub_path,
+    thrust_path,
+    libcudacxx_path,
+    ctk_path,
+    config);
+  if (r != CUDA_SUCCESS)
+  {
+    return r;
+  }
+  r = cccl_device_segmented_sort_load(build);
+  if (r != CUDA_SUCCESS)
+  {
+    free_build(build);
+  }
+  return r;
+}
+
+CCCL_C_API CUresult cccl_device_segmented_sort_build(
+  BuildResult* build,
+  cccl_sort_order_t sort_order,
+  cccl_iterator_t d_keys_in,
+  cccl_iterator_t d_values_in,
+  cccl_iterator_t begin_offset_in,
+  cccl_iterator_t end_offset_in,
+  int cc_major,
+  int cc_minor,
+  const char* cub_path,
+  const char* thrust_path,
+  const char* libcudacxx_path,
+  const char* ctk_path)
+{
+  return cccl_device_segmented_sort_build_ex(
+    build,
+    sort_order,
+    d_keys_in,
+    d_values_in,
+    begin_offset_in,
+    end_offset_in,
+    cc_major,
+    cc_minor,
+    cub_path,
+    thrust_path,
+    libcudacxx_path,
+    ctk_path,
+    nullptr);
+}
+
+CCCL_C_API CUresult cccl_device_segmented_sort(
+  BuildResult build,
+  void* d_temp_storage,
+  size_t* temp_storage_bytes,
+  cccl_iterator_t d_keys_in,
+  cccl_iterator_t d_keys_out,
+  cccl_iterator_t d_values_in,
+  cccl_iterator_t d_values_out,
+  uint64_t num_items,
+  uint64_t num_segments,
+  cccl_iterator_t start_offset_in,
+  cccl_iterator_t end_offset_in,
+  bool is_overwrite_okay,
+  int* selector,
+  CUstream stream)
+{
+  if (d_keys_in.type != CCCL_POINTER || d_keys_out.type != CCCL_POINTER || d_values_in.type != CCCL_POINTER
+      || d_values_out.type != CCCL_POINTER || start_offset_in.type != CCCL_POINTER
+      || end_offset_in.type != CCCL_POINTER)
+  {
+    std::fflush(stderr);
+    std::printf("\nERROR in cccl_device_segmented_sort(): keys, values and offsets must be pointers\n");
+    std::fflush(stdout);
+    return CUDA_ERROR_UNKNOWN;
+  }
+  if (temp_storage_bytes == nullptr || selector == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  SegSortPayload sp{};
+  if (!read_payload(build, sp) || !policies_ok(build) || build.segmented_sort_kernel_large == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  if (num_segments > uint64_t(0x7fffffff))
+  {
+    cccl_amd::set_last_error("segmented sort: %llu segments, at most 2^31 - 1 are supported",
+                             static_cast<unsigned long long>(num_segments));
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  if (sp.offset_bytes == 4 && sp.offset_unsigned && num_items >= (uint64_t(1) << 31))
+  {
+    cccl_amd::set_last_error("segmented sort: unsigned 32-bit offsets need fewer than 2^31 items");
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  try
+  {
+    SortProblem pr{};
+    pr.key_bytes  = sp.key_bytes;
+    pr.key_kind   = sp.key_kind;
+    pr.val_bytes  = sp.value_bytes;
+    pr.num_items  = num_items;
+    pr.begin_bit  = 0;
+    pr.end_bit    = sp.key_bytes * 8;
+    pr.descending = sp.order == CCCL_DESCENDING;
+    pr.overwrite  = is_overwrite_okay;
+    const cccl_amd_segmented_config_t cfg = *static_cast<const cccl_amd_segmented_config_t*>(build.runtime_policy);
+    DoubleBufferRef keys{{d_keys_in.state, d_keys_out.state}, 0};
+    DoubleBufferRef vals{{d_values_in.state, d_values_out.state}, 0};
+    if (pr.val_bytes == 0)
+    {
+      vals = DoubleBufferRef{{nullptr, nullptr}, 0};
+    }
+    const hipError_t e = cccl_amd::segmented_sort(
+      d_temp_storage,
+      *temp_storage_bytes,
+      keys,
+      vals,
+      pr,
+      int64_t(num_segments),
+      start_offset_in.state,
+      end_offset_in.state,
+      sp.offset_bytes,
+      cfg,
+      reinterpret_cast<hipStream_t>(stream));
+    *selector = keys.selector;
+    return static_cast<CUresult>(e);
+  }
+  catch (const std::exception& exc)
+  {
+    std::fflush(stderr);
+    std::printf("\nEXCEPTION in cccl_device_segmented_sort(): %s\n", exc.what());
+    std::fflush(stdout);
+    return CUDA_ERROR_UNKNOWN;
+  }
+}
+
+CCCL_C_API CUresult cccl_device_segmented_sort_link_ltoir(
+  BuildResult* build, const void** /*input_blobs*/, const size_t* /*input_sizes*/, size_t num_inputs)
+{
+  if (build == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  return num_inputs == 0 ? CUDA_SUCCESS : CUDA_ERROR_NOT_SUPPORTED;
+}
+
+CCCL_C_API CUresult cccl_device_segmented_sort_serialize(const BuildResult* build, void** out_buf, size_t* out_size)
+{
+  if (build == nullptr || out_buf == nullptr || out_size == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  if (build->payload == nullptr || build->payload_size == 0 || !policies_ok(*build))
+  {
+    *out_buf  = nullptr;
+    *out_size = 0;
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  try
+  {
+    Writer w;
+    w.bytes(kBlobMagic, 8);
+    w.pod<int32_t>(build->cc);
+    w.pod<uint32_t>(uint32_t(build->payload_kind));
+    w.type_info(build->key_type);
+    w.type_info(build->offset_type);
+    w.pod<uint32_t>(uint32_t(build->order));
+    w.blob(build->payload, build->payload_size);
+    w.blob(build->runtime_policy, build->runtime_policy_size);
+    w.blob(build->partition_runtime_policy, build->partition_runtime_policy_size);
+    for (int i = 0; i < kNumNames; ++i)
+    {
+      w.cstr(*name_slot(const_cast<BuildResult*>(build), i));
+    }
+    void* out = std::malloc(w.buf.size());
+    if (out == nullptr)
+    {
+      return CUDA_ERROR_OUT_OF_MEMORY;
+    }
+    std::memcpy(out, w.buf.data(), w.buf.size());
+    *out_buf  = out;
+    *out_size = w.buf.size();
+    return CUDA_SUCCESS;
+  }
+  catch (const std::exception& exc)
+  {
+    std::printf("\nEXCEPTION in cccl_device_segmented_sort_serialize(): %s\n", exc.what());
+    std::fflush(stdout);
+    return CUDA_ERROR_UNKNOWN;
+  }
+}
+
+CCCL_C_API CUresult cccl_device_segmented_sort_deserialize(BuildResult* build, const void* buf, size_t size)
+{
+  if (build == nullptr || buf == nullptr || size == 0)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  BuildResult r{};
+  char* policy  = nullptr;
+  char* ppolicy = nullptr;
+  try
+  {
+    Reader rd{static_cast<const unsigned char*>(buf), size};
+    char magic[8] = {};
+    rd.bytes(magic, 8);
+    if (!rd.ok || std::memcmp(magic, kBlobMagic, 8) != 0)
+    {
+      return CUDA_ERROR_INVALID_VALUE;
+    }
+    r.cc           = rd.pod<int32_t>();
+    r.payload_kind = cccl_payload_kind_t(rd.pod<uint32_t>());
+    r.key_type     = rd.type_info();
+    r.offset_type  = rd.type_info();
+    r.order        = cccl_sort_order_t(rd.pod<uint32_t>());
+    if (!rd.ok || (r.order != CCCL_ASCENDING && r.order != CCCL_DESCENDING))
+    {
+      return CUDA_ERROR_INVALID_VALUE;
+    }
+    r.payload = rd.blob_new(r.payload_size);
+    size_t policy_size = 0, ppolicy_size = 0;
+    policy  = rd.ok ? rd.blob_new(policy_size) : nullptr;
+    ppolicy = rd.ok ? rd.blob_new(ppolicy_size) : nullptr;
+    for (int i = 0; i < kNumNames && rd.ok; ++i)
+    {
+      *name_slot(&r, i) = rd.cstr_new();
+    }
+    SegSortPayload sp{};
+    if (!rd.ok || r.payload_size == 0 || !read_payload(r, sp) || policy_size != sizeof(cccl_amd_segmented_config_t)
+        || ppolicy_size != sizeof(cccl_amd_segment_partition_policy_t))
+    {
+      delete[] policy;
+      delete[] ppolicy;
+      free_build(&r);
+      return CUDA_ERROR_INVALID_VALUE;
+    }
+    r.runtime_policy           = std::malloc(policy_size);
+    r.partition_runtime_policy = std::malloc(ppolicy_size);
+    if (r.runtime_policy == nullptr || r.partition_runtime_policy == nullptr)
+    {
+      delete[] policy;
+      delete[] ppolicy;
+      free_build(&r);
+      return CUDA_ERROR_OUT_OF_MEMORY;
+    }
+    std::memcpy(r.runtime_policy, policy, policy_size);
+    std::memcpy(r.partition_runtime_policy, ppolicy, ppolicy_size);
+    r.runtime_policy_size           = policy_size;
+    r.partition_runtime_policy_size = ppolicy_size;
+    delete[] policy;
+    delete[] ppolicy;
+    *build = r;
+    return CUDA_SUCCESS;
+  }
+  catch (const std::exception& exc)
+  {
+    delete[] policy;
+    delete[] ppolicy;
+    free_build(&r);
+    std::printf("\nEXCEPTION in cccl_device_segmented_sort_deserialize(): %s\n", exc.what());
+    std::fflush(stdout);
+    return CUDA_ERROR_UNKNOWN;
+  }
+}
+
+CCCL_C_API CUresult cccl_device_segmented_sort_cleanup(BuildResult* bld_ptr)
+{
+  if (bld_ptr == nullptr)
+  {
+    return CUDA_ERROR_INVALID_VALUE;
+  }
+  free_build(bld_ptr);
+  return CUDA_SUCCESS;
+}
+
+} // extern "C"

@@ -1,0 +1,276 @@
+"""cub::DeviceSegmentedSort C-ABI checks that need no GPU, after
+c/parallel/test/test_segmented_sort.cpp:728-881 (build metadata, compile/load round
+trip) and the error paths of c/parallel/src/segmented_sort.cu; plus the oracle pinned to
+DeviceSegmentedSort's semantics: each segment ordered like std::stable_sort with `<`
+or `>` on the key values (test_segmented_sort.cpp:425-436), which a per-segment numpy
+stable argsort restates independently of the oracle's bit transform."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from cccl_amd import _bindings as B
+from sortutil import float_keys_with_specials, random_keys
+
+SEG_ENTRY_POINTS = [
+    "cccl_device_segmented_sort_build", "cccl_device_segmented_sort_build_ex", "cccl_device_segmented_sort_compile",
+    "cccl_device_segmented_sort_load", "cccl_device_segmented_sort", "cccl_device_segmented_sort_link_ltoir",
+    "cccl_device_segmented_sort_serialize", "cccl_device_segmented_sort_deserialize",
+    "cccl_device_segmented_sort_cleanup",
+]
+
+
+def _its(key_type=B.CCCL_INT32, key_size=4, value_size=4, off_type=B.CCCL_INT32, off_size=4, off_kind=B.CCCL_POINTER):
+    kit = B.pointer_iterator(key_type, key_size, None)
+    vit = B.pointer_iterator(B.CCCL_STORAGE, value_size, 1 if value_size else None)
+    oit = B.pointer_iterator(off_type, off_size, None)
+    oit.type = off_kind
+    return kit, vit, oit
+
+
+def _build(order=B.CCCL_ASCENDING, cc=(9, 5), fn="cccl_device_segmented_sort_build", **kw):
+    L = B.lib()
+    br = B.cccl_device_segmented_sort_build_result_t()
+    kit, vit, oit = _its(**kw)
+    args = [ctypes.byref(br), order, kit, vit, oit, oit, cc[0], cc[1], None, None, None, None]
+    if fn != "cccl_device_segmented_sort_build":
+        args.append(None)
+    rc = getattr(L, fn)(*args)
+    return rc, br
+
+
+def test_entry_points_exported():
+    L = B.lib()
+    for name in SEG_ENTRY_POINTS:
+        assert name in B.EXPORTED_SYMBOLS
+        assert hasattr(L, name), name
+
+
+def test_build_result_metadata_populated():
+    """test_segmented_sort.cpp:728-774."""
+    L = B.lib()
+    rc, br = _build()
+    assert rc == B.CUDA_SUCCESS
+    assert br.cc == 95
+    assert br.payload and br.payload_kind == 1 and br.payload_size > 0  # CCCL_PAYLOAD_CUBIN
+    assert br.runtime_policy and br.runtime_policy_size > 0
+    assert br.partition_runtime_policy and br.partition_runtime_policy_size > 0
+    for n in B._SEG_NAMES:
+        name = getattr(br, n + "_lowered_name")
+        assert name is not None and len(name) > 0, n
+        assert getattr(br, n), n
+    assert br.library
+    assert br.key_type.type == B.CCCL_INT32 and br.offset_type.type == B.CCCL_INT32
+    pp = ctypes.cast(br.partition_runtime_policy, ctypes.POINTER(B.segment_partition_policy)).contents
+    cfg = ctypes.cast(br.runtime_policy, ctypes.POINTER(B.segmented_config)).contents
+    assert pp.num_classes == cfg.num_block_classes > 0
+    bounds = [pp.class_max_items[c] for c in range(pp.num_classes)]
+    assert bounds == sorted(bounds) and len(set(bounds)) == len(bounds)
+    assert bounds == [cfg.block_threads[c] * cfg.block_ipt[c] for c in range(cfg.num_block_classes)]
+    assert pp.large_tile_items == cfg.large_threads * cfg.large_ipt
+    assert L.cccl_device_segmented_sort_cleanup(ctypes.byref(br)) == B.CUDA_SUCCESS
+    assert not br.payload and not br.runtime_policy and not br.partition_runtime_policy
+
+
+def test_compile_then_load():
+    """test_segmented_sort.cpp:776-812: compile leaves the handles null, load resolves them."""
+    L = B.lib()
+    rc, br = _build(fn="cccl_device_segmented_sort_compile")
+    assert rc == B.CUDA_SUCCESS
+    assert br.payload and br.segmented_sort_fallback_kernel_lowered_name
+    assert not br.library and not br.segmented_sort_fallback_kernel
+    assert L.cccl_device_segmented_sort_load(ctypes.byref(br)) == B.CUDA_SUCCESS
+    assert br.library and br.segmented_sort_fallback_kernel and br.segmented_sort_kernel_small
+    assert br.three_way_partition_init_kernel and br.three_way_partition_kernel and br.segmented_sort_kernel_large
+    L.cccl_device_segmented_sort_cleanup(ctypes.byref(br))
+
+
+def test_build_ex_and_keys_only():
+    L = B.lib()
+    rc, br = _build(fn="cccl_device_segmented_sort_build_ex", value_size=0, key_type=B.CCCL_FLOAT64, key_size=8,
+                    off_type=B.CCCL_INT64, off_size=8, order=B.CCCL_DESCENDING)
+    assert rc == B.CUDA_SUCCESS and br.order == B.CCCL_DESCENDING and br.offset_type.size == 8
+    L.cccl_device_segmented_sort_cleanup(ctypes.byref(br))
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(cc=(9, 0)), B.CUDA_ERROR_NOT_SUPPORTED),
+    (dict(off_kind=B.CCCL_ITERATOR), B.CUDA_ERROR_NOT_SUPPORTED),
+    (dict(off_type=B.CCCL_INT16, off_size=2), B.CUDA_ERROR_NOT_SUPPORTED),
+    (dict(value_size=3), B.CUDA_ERROR_NOT_SUPPORTED),
+    (dict(key_type=B.CCCL_STORAGE, key_size=4), B.CUDA_ERROR_NOT_SUPPORTED),
+    (dict(order=7), B.CUDA_ERROR_INVALID_VALUE),
+])
+def test_build_rejections(kw, code):
+    rc, br = _build(**kw)
+    assert rc == code
+    assert not br.payload
+
+
+def test_null_and_incomplete_build_results():
+    L = B.lib()
+    assert L.cccl_device_segmented_sort_cleanup(None) == B.CUDA_ERROR_INVALID_VALUE
+    assert L.cccl_device_segmented_sort_load(None) == B.CUDA_ERROR_INVALID_VALUE
+    empty = B.cccl_device_segmented_sort_build_result_t()
+    assert L.cccl_device_segmented_sort_load(ctypes.byref(empty)) == B.CUDA_ERROR_INVALID_VALUE
+    buf, size = ctypes.c_void_p(), ctypes.c_size_t()
+    assert L.cccl_device_segmented_sort_serialize(ctypes.byref(empty), ctypes.byref(buf),
+                                                  ctypes.byref(size)) == B.CUDA_ERROR_INVALID_VALUE
+    assert L.cccl_device_segmented_sort_link_ltoir(None, None, None, 0) == B.CUDA_ERROR_INVALID_VALUE
+    rc, br = _build()
+    assert L.cccl_device_segmented_sort_link_ltoir(ctypes.byref(br), None, None, 0) == B.CUDA_SUCCESS
+    assert L.cccl_device_segmented_sort_link_ltoir(ctypes.byref(br), None, None, 1) == B.CUDA_ERROR_NOT_SUPPORTED
+    L.cccl_device_segmented_sort_cleanup(ctypes.byref(br))
+
+
+def test_serialize_round_trip_then_load():
+    L = B.lib()
+    rc, br = _build(key_type=B.CCCL_UINT16, key_size=2, value_size=8, order=B.CCCL_DESCENDING)
+    assert rc == B.CUDA_SUCCESS
+    buf, size = ctypes.c_void_p(), ctypes.c_size_t()
+    assert L.cccl_device_segmented_sort_serialize(ctypes.byref(br), ctypes.byref(buf),
+                                                  ctypes.byref(size)) == B.CUDA_SUCCESS
+    blob = ctypes.string_at(buf, size.value)
+    L.cccl_serialization_buffer_free(buf)
+    assert blob[:8] == b"CCAMDSS1"
+    r2 = B.cccl_device_segmented_sort_build_result_t()
+    assert L.cccl_device_segmented_sort_deserialize(ctypes.byref(r2), blob, len(blob)) == B.CUDA_SUCCESS
+    assert not r2.library and not r2.segmented_sort_kernel_large  # handles come from _load
+    assert r2.order == B.CCCL_DESCENDING and r2.key_type.type == B.CCCL_UINT16 and r2.cc == 95
+    assert ctypes.string_at(r2.payload, r2.payload_size) == ctypes.string_at(br.payload, br.payload_size)
+    assert ctypes.string_at(r2.runtime_policy, r2.runtime_policy_size) == \
+        ctypes.string_at(br.runtime_policy, br.runtime_policy_size)
+    assert ctypes.string_at(r2.partition_runtime_policy, r2.partition_runtime_policy_size) == \
+        ctypes.string_at(br.partition_runtime_policy, br.partition_runtime_policy_size)
+    for n in B._SEG_NAMES:
+        assert getattr(r2, n + "_lowered_name") == getattr(br, n + "_lowered_name")
+    assert L.cccl_device_segmented_sort_load(ctypes.byref(r2)) == B.CUDA_SUCCESS
+    L.cccl_device_segmented_sort_cleanup(ctypes.byref(r2))
+    L.cccl_device_segmented_sort_cleanup(ctypes.byref(br))
+
+
+def test_deserialize_rejects_bad_blobs_and_leaves_build_unchanged():
+    L = B.lib()
+    rc, br = _build()
+    buf, size = ctypes.c_void_p(), ctypes.c_size_t()
+    L.cccl_device_segmented_sort_serialize(ctypes.byref(br), ctypes.byref(buf), ctypes.byref(size))
+    blob = ctypes.string_at(buf, size.value)
+    L.cccl_serialization_buffer_free(buf)
+    target = B.cccl_device_segmented_sort_build_result_t()
+    target.cc = 1234
+    for bad in (b"garbage!" + blob[8:], blob[: len(blob) // 2], blob[:9]):
+        assert L.cccl_device_segmented_sort_deserialize(ctypes.byref(target), bad, len(bad)) == \
+            B.CUDA_ERROR_INVALID_VALUE
+        assert target.cc == 1234 and not target.payload
+    # a radix-sort blob is not a segmented-sort blob
+    rrc = B.cccl_device_radix_sort_build_result_t()
+    kit = B.pointer_iterator(B.CCCL_UINT32, 4, None)
+    L.cccl_device_radix_sort_build(ctypes.byref(rrc), 0, kit, B.pointer_iterator(B.CCCL_STORAGE, 0, None),
+                                   B.identity_op(), None, 9, 5, None, None, None, None)
+    L.cccl_device_radix_sort_serialize(ctypes.byref(rrc), ctypes.byref(buf), ctypes.byref(size))
+    rblob = ctypes.string_at(buf, size.value)
+    L.cccl_serialization_buffer_free(buf)
+    assert L.cccl_device_segmented_sort_deserialize(ctypes.byref(target), rblob, len(rblob)) == \
+        B.CUDA_ERROR_INVALID_VALUE
+    L.cccl_device_radix_sort_cleanup(ctypes.byref(rrc))
+    L.cccl_device_segmented_sort_cleanup(ctypes.byref(br))
+
+
+def _run(br, temp, nbytes, n, nseg, overwrite=False, key_kind=B.CCCL_POINTER, off_type=B.CCCL_INT32, off_size=4,
+         fake=0x1000):
+    L = B.lib()
+    k = B.pointer_iterator(br.key_type.type, br.key_type.size, fake)
+    k.type = key_kind
+    v = B.pointer_iterator(B.CCCL_STORAGE, 4, fake)
+    o = B.pointer_iterator(off_type, off_size, fake)
+    sel = ctypes.c_int(-1)
+    rc = L.cccl_device_segmented_sort(br, temp, ctypes.byref(nbytes), k, k, v, v, n, nseg, o, o, overwrite,
+                                      ctypes.byref(sel), None)
+    return rc, sel.value
+
+
+def test_temp_storage_query_launches_nothing():
+    """d_temp_storage == NULL only sizes the temp storage (fake device pointers are never
+    touched; there is no GPU here)."""
+    L = B.lib()
+    sizes = {}
+    for key_type, ks in ((B.CCCL_UINT32, 4), (B.CCCL_UINT16, 2), (B.CCCL_UINT8, 1), (B.CCCL_FLOAT64, 8)):
+        rc, br = _build(key_type=key_type, key_size=ks)
+        assert rc == B.CUDA_SUCCESS
+        for ow in (False, True):
+            nbytes = ctypes.c_size_t(0)
+            rc, sel = _run(br, None, nbytes, 1 << 20, 1000, overwrite=ow)
+            assert rc == B.CUDA_SUCCESS and nbytes.value > 0 and sel == 0
+            sizes[ks, ow] = nbytes.value
+        nbytes = ctypes.c_size_t(0)
+        assert _run(br, None, nbytes, 0, 0)[0] == B.CUDA_SUCCESS and nbytes.value > 0
+        L.cccl_device_segmented_sort_cleanup(ctypes.byref(br))
+    # is_overwrite_okay drops the extra key + value buffer where the final buffer parity
+    # agrees (4-byte keys: 6 CUB passes, 4 here) or is reached by a copy (1-byte keys);
+    # 2- and 8-byte keys (3 and 11 CUB passes) sort into the alternate buffer and keep it.
+    for ks in (4, 1):
+        assert sizes[ks, False] - sizes[ks, True] >= (1 << 20) * (ks + 4)
+    for ks in (2, 8):
+        assert sizes[ks, True] == sizes[ks, False]
+
+
+def test_run_rejections():
+    rc, br = _build()
+    nbytes = ctypes.c_size_t(0)
+    assert _run(br, None, nbytes, 100, 3, key_kind=B.CCCL_ITERATOR)[0] == B.CUDA_ERROR_UNKNOWN
+    assert _run(br, None, nbytes, 100, 2**31)[0] == B.CUDA_ERROR_INVALID_VALUE
+    empty = B.cccl_device_segmented_sort_build_result_t()
+    assert _run(empty, None, nbytes, 100, 3)[0] == B.CUDA_ERROR_INVALID_VALUE
+    B.lib().cccl_device_segmented_sort_cleanup(ctypes.byref(br))
+    rc, br = _build(off_type=B.CCCL_UINT32)
+    assert _run(br, None, nbytes, 2**31, 3, off_type=B.CCCL_UINT32)[0] == B.CUDA_ERROR_INVALID_VALUE
+    assert _run(br, None, nbytes, 2**31 - 1, 3, off_type=B.CCCL_UINT32)[0] == B.CUDA_SUCCESS
+    B.lib().cccl_device_segmented_sort_cleanup(ctypes.byref(br))
+
+
+# ---- oracle pinned to DeviceSegmentedSort's comparison semantics -------------
+
+def _comparison_stable(keys, seg_begin, seg_end, descending):
+    """Per segment: numpy stable argsort of the key values themselves (`<`), or of an
+    order-reversing image of them for `>` (negation for floats, bitwise NOT for
+    integers), the order std::stable_sort with std::less / std::greater gives."""
+    perm = np.arange(keys.size)
+    for b, e in zip(seg_begin, seg_end):
+        seg = keys[b:e]
+        if descending:
+            seg = -seg if seg.dtype.kind == "f" else ~seg
+        perm[b:e] = b + np.argsort(seg, kind="stable")
+    return perm
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.uint16, np.int16, np.float16, np.uint32, np.int32,
+                                   np.float32, np.uint64, np.int64, np.float64])
+@pytest.mark.parametrize("descending", [False, True])
+def test_oracle_matches_comparison_sort_per_segment(dtype, descending):
+    rng = np.random.default_rng(11 + np.dtype(dtype).itemsize)
+    sizes = np.array([1, 5, 10, 20, 30, 50, 100, 3, 25, 600, 7, 18, 300, 4, 35, 9, 14, 700, 28, 11] * 3)
+    n = int(sizes.sum())
+    if np.dtype(dtype).kind == "f":
+        keys = float_keys_with_specials(rng, dtype, n, zero_frac=0.05, inf_frac=0.01)
+    else:
+        keys = random_keys(rng, dtype, n, "and3")  # many equal keys: stability shows
+    ends = np.cumsum(sizes)
+    begins = ends - sizes
+    vals = np.arange(n, dtype=np.uint32)
+    ek, ev = oracle.segmented_sort(keys, vals, begins, ends, keys, vals, descending=descending)
+    perm = _comparison_stable(keys, begins, ends, descending)
+    assert np.array_equal(ev, vals[perm])
+    assert np.array_equal(ek.view(f"u{keys.itemsize}"), keys[perm].view(f"u{keys.itemsize}"))  # -0.0 bits kept
+
+
+def test_oracle_comparison_pin_catches_a_reversed_tie_order():
+    """A plausible mistake, equal keys emitted in reverse input order, fails the pin."""
+    keys = np.array([3, 1, 3, 1, 2], np.int32)
+    vals = np.arange(5, dtype=np.uint32)
+    perm = _comparison_stable(keys, [0], [5], descending=True)
+    assert list(vals[perm]) == [0, 2, 4, 1, 3]
+    _, ev = oracle.segmented_sort(keys, vals, [0], [5], keys, vals, descending=True)
+    assert list(ev) == [0, 2, 4, 1, 3]
+    assert list(ev) != [2, 0, 4, 3, 1]
