@@ -277,6 +277,22 @@ def leg_segmented(torch, ca, dev, steps):
     ms_default = time_steps(torch, lambda: s.sort(kin, kout, begins, ends, vin, vout), steps, 1)
     del s
 
+    # the same batch through cub::DeviceSegmentedSort's C-ABI (cccl_device_segmented_sort):
+    # pointer form timed; the DoubleBuffer form once on copies (it sorts into the input)
+    dss = ca.make_segmented_sort(d_in_keys=kin, d_out_keys=kout, d_in_values=vin, d_out_values=vout,
+                                 start_offsets_in=begins, end_offsets_in=ends, order=ca.SortOrder.ASCENDING)
+    dss_kw = dict(d_in_keys=kin, d_out_keys=kout, d_in_values=vin, d_out_values=vout, num_items=n,
+                  num_segments=nseg, start_offsets_in=begins, end_offsets_in=ends)
+    dss_temp = torch.empty(max(dss(temp_storage=None, **dss_kw), 1), dtype=torch.uint8, device=dev)
+    kout.zero_()
+    ms_dss = time_steps(torch, lambda: dss(temp_storage=dss_temp, **dss_kw), steps, 1)
+    dss_ok = bool(torch.equal(kout, ref_k) and torch.equal(vout, ref_v))
+    kdb, vdb = ca.DoubleBuffer(kin.clone(), torch.empty_like(kin)), ca.DoubleBuffer(vin.clone(), torch.empty_like(vin))
+    dss(temp_storage=dss_temp, **dict(dss_kw, d_in_keys=kdb, d_out_keys=None, d_in_values=vdb, d_out_values=None))
+    torch.cuda.synchronize()
+    dss_db_ok = bool(kdb.selector == 0 and torch.equal(kdb.current(), ref_k) and torch.equal(vdb.current(), ref_v))
+    del dss, dss_temp, kdb, vdb
+
     # CUB's DeviceSegmentedRadixSort schedule: one workgroup per segment per pass
     cub_arm = S.FIXED_DEFAULT["segmented"]
     s, fn = run(S.ARMS[cub_arm])
@@ -315,6 +331,8 @@ def leg_segmented(torch, ca, dev, steps):
         "batch_ms": round(ms_batch, 4), "batch_gkeys_per_s": round(n / ms_batch / 1e6, 3),
         "hipgraph_replay_ms": round(ms_graph, 4), "hipgraph_replay_ok": graph_ok,
         "library_default_ms": round(ms_default, 4),
+        "device_segmented_sort_ms": round(ms_dss, 4), "device_segmented_sort_matches_batch": dss_ok,
+        "device_segmented_sort_doublebuffer_ok": dss_db_ok,
         "cub_schedule_ms": round(ms_cub, 4), "cub_schedule_config": S.ARMS[cub_arm].name,
         "per_segment_loop_ms_scaled": round(ms_loop, 2),
         "per_segment_loop_sample": f"{sample.size} random segments timed ({ms_loop_sample:.2f} ms), x {nseg / sample.size:.0f}",
