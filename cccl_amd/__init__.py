@@ -26,3 +26,4 @@ from .radix_sort import (  # noqa: F401
     segmented_radix_sort,
 )
 from .segmented_sort import make_segmented_sort, segmented_sort  # noqa: F401,E402
+from . import topk  # noqa: F401,E402

@@ -213,6 +213,7 @@ EXPORTED_SYMBOLS = [
     "cccl_amd_mlp_workspace_bytes",
     "cccl_amd_mlp_forward",
     "cccl_amd_mlp_train_step",
+    "cccl_amd_topk",
 ]
 
 _lib = None
@@ -315,6 +316,8 @@ def lib():
     L.cccl_amd_mlp_forward.restype = i32
     L.cccl_amd_mlp_train_step.argtypes = [fp, fp, fp, fp, fp, fp, sz, fp, fp, fp, i32, i32, ctypes.c_float, i32, vp]
     L.cccl_amd_mlp_train_step.restype = i32
+    L.cccl_amd_topk.argtypes = [vp, ctypes.POINTER(sz), vp, vp, vp, vp, i32, sz, u64, u64, i32, vp]
+    L.cccl_amd_topk.restype = i32
     _lib = L
     return L
 

@@ -28,6 +28,7 @@ SOURCES = [
     "segmented.hip",
     "capi.hip",
     "segmented_sort_capi.hip",
+    "topk.hip",
     "graph.hip",
     "mlp.hip",
     "features.hip",

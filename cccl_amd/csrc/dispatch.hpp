@@ -84,6 +84,23 @@ hipError_t segmented_sort(
   const cccl_amd_segmented_config_t& cfg,
   hipStream_t stream);
 
+// cub::DeviceTopK: the k items of smallest (or, select_max, largest) twiddled key,
+// unordered; k is capped to num_items.
+hipError_t topk(
+  void* d_temp_storage,
+  size_t& temp_storage_bytes,
+  const void* d_keys_in,
+  void* d_keys_out,
+  const void* d_values_in,
+  void* d_values_out,
+  int key_bytes,
+  int key_kind,
+  int val_bytes,
+  uint64_t num_items,
+  uint64_t k,
+  bool select_max,
+  hipStream_t stream);
+
 cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64_t num_items);
 
 bool config_is_valid(const cccl_amd_radix_sort_config_t& cfg, int key_bytes, int val_bytes);

@@ -6,6 +6,9 @@ package, and only as the checker / the baseline being timed.  The product path
 
 * liboracle.so (oracle.cpp): closed-form restatement of CUB's test oracle
   (cub/test/catch2_radix_sort_helper.cuh:174-312).
+* topk_order_bits / topk (numpy): the order of cub::DeviceTopK
+  (cub/cub/device/dispatch/dispatch_topk.cuh:57-82) and the multiset of order images
+  the k selected keys must have.
 * _ref/libthrust_ref.so (thrust_ref.cpp): the reference's own Thrust OpenMP
   stable_sort[_by_key] (thrust/thrust/system/omp/detail/sort.h), compiled from
   /root/reference by oracle/Makefile.  Absent where the reference tree is absent
@@ -195,3 +198,32 @@ def ref_sort_inplace(keys: np.ndarray, values: np.ndarray | None = None, descend
 
 def ref_max_threads() -> int:
     return _load_ref().ref_max_threads()
+
+
+def topk_order_bits(keys: np.ndarray, select_max: bool) -> np.ndarray:
+    """Order image of each key for cub::DeviceTopK, as uint64
+    (cub/cub/device/dispatch/dispatch_topk.cuh:57-82): Traits<T>::TwiddleIn on the key
+    bits -- unsigned identity, signed sign flip, floats all bits flipped when negative
+    and the sign bit set otherwise, with no -0.0 / +0.0 merging -- then NOT for Max.
+    DeviceTopK returns the k keys of smallest image."""
+    keys = np.ascontiguousarray(keys)
+    kb, kd = key_kind(keys.dtype)
+    u = keys.view(f"u{kb}").astype(np.uint64)
+    bits = 8 * kb
+    mask = np.uint64((1 << bits) - 1)
+    high = np.uint64(1 << (bits - 1))
+    if kd == KIND_FLOAT:
+        u = np.where((u & high) != 0, ~u & mask, u | high)
+    elif kd == KIND_SIGNED:
+        u = u ^ high
+    if select_max:
+        u = ~u & mask
+    return u
+
+
+def topk(keys: np.ndarray, k: int, select_max: bool) -> np.ndarray:
+    """Sorted order images of the k selected keys (k capped to the key count): every
+    valid top-k output's keys have exactly these images, whichever tied items it took."""
+    o = topk_order_bits(keys, select_max)
+    k = min(int(k), o.size)
+    return np.sort(np.partition(o, k - 1)[:k]) if k > 0 else o[:0]

@@ -1,0 +1,187 @@
+"""GPU parity of cub::DeviceTopK (cccl_amd_topk through cccl_amd.topk) against the oracle.
+
+Follows cub/test/catch2_test_device_topk_keys.cu and _pairs.cu: every key type, Max and
+Min, random k in [1, num_items] plus k = num_items and k > num_items (capped), sizes from
+1 item to 2^24, low-entropy keys (many ties at the k-th key), and keys whose selection
+completes before the last digit.  The output is unordered and, at a tie with the k-th
+key, any tied items may be returned (device_topk.cuh:86-92), so the checks are:
+* the sorted order images of the output keys equal the oracle's (a unique multiset);
+* pairs (values = input index): every value indexes an input item whose key is the
+  output key bit for bit, no index repeats, and every item strictly better than the
+  k-th key is present.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from sortutil import float_keys_with_specials, random_keys, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def gen(rng, dtype, n, entropy="uniform"):
+    dt = np.dtype(dtype)
+    if dt.kind == "f":
+        k = float_keys_with_specials(rng, dt, n, zero_frac=0.02, inf_frac=0.005)
+        if n >= 8:  # a few NaNs of both signs: they rank by bit pattern
+            k[rng.integers(0, n, 4)] = np.nan
+            k[rng.integers(0, n, 2)] = -np.nan
+        if entropy != "uniform":
+            k = np.round(k * 2) / 2  # many equal keys
+        return k.astype(dt)
+    return random_keys(rng, dt, n, entropy)
+
+
+def run(keys, k, select_max, with_values, value_dtype=np.uint32):
+    import cccl_amd as ca
+
+    n = keys.size
+    kd = to_dev(keys)
+    kcap = min(k, n)
+    ko = to_dev(np.full(max(kcap, 1) * keys.itemsize, 0x5A, np.uint8)).view(kd.dtype)
+    if with_values:
+        vals = np.arange(n, dtype=value_dtype)
+        vd = to_dev(vals)
+        vo = to_dev(np.full(max(kcap, 1) * vals.itemsize, 0xA5, np.uint8)).view(vd.dtype)
+        fn = ca.topk.max_pairs if select_max else ca.topk.min_pairs
+        fn(kd, ko, vd, vo, n, k)
+    else:
+        fn = ca.topk.max_keys if select_max else ca.topk.min_keys
+        fn(kd, ko, n, k)
+    torch.cuda.synchronize()
+    out_k = to_host(ko, keys.dtype)[:kcap]
+    out_v = to_host(vo, value_dtype)[:kcap] if with_values else None
+    return out_k, out_v
+
+
+def check(keys, k, select_max, with_values=True, value_dtype=np.uint32):
+    out_k, out_v = run(keys, k, select_max, with_values, value_dtype)
+    kcap = min(k, keys.size)
+    expect = oracle.topk(keys, k, select_max)
+    got = np.sort(oracle.topk_order_bits(out_k, select_max))
+    assert np.array_equal(got, expect), f"key multiset differs (n={keys.size}, k={k}, max={select_max})"
+    if with_values:
+        idx = out_v.astype(np.int64)
+        assert idx.min(initial=0) >= 0 and idx.max(initial=0) < keys.size
+        assert np.unique(idx).size == kcap, "an item was returned twice"
+        u = f"u{keys.itemsize}"
+        assert np.array_equal(keys[idx].view(u), out_k.view(u)), "a value does not belong to its key"
+        if kcap > 0:
+            o = oracle.topk_order_bits(keys, select_max)
+            better = np.flatnonzero(o < expect[-1])
+            assert np.isin(better, idx).all(), "an item better than the k-th key is missing"
+
+
+DTYPES = [np.uint8, np.int8, np.uint16, np.int16, np.float16, np.uint32, np.int32, np.float32, np.uint64, np.int64,
+          np.float64]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("select_max", [True, False])
+def test_keys_random_k(dtype, select_max):
+    rng = np.random.default_rng(100 + np.dtype(dtype).itemsize + select_max)
+    for n in (1, 3, 1000, (1 << 20) + 7):
+        keys = gen(rng, dtype, n)
+        for k in sorted({1, int(rng.integers(1, n + 1)), int(rng.integers(1, n + 1)), n}):
+            check(keys, k, select_max, with_values=False)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.float32, np.uint64, np.float64])
+@pytest.mark.parametrize("select_max", [True, False])
+@pytest.mark.parametrize("entropy", ["uniform", "and3"])
+def test_pairs_ties_and_membership(dtype, select_max, entropy):
+    rng = np.random.default_rng(7)
+    n = 300_000
+    keys = gen(rng, dtype, n, entropy)
+    for k in (1, 100, 4096, int(rng.integers(1, n)), n - 1):
+        check(keys, k, select_max)
+
+
+@pytest.mark.parametrize("value_dtype", [np.uint8, np.uint16, np.uint64])
+def test_value_widths(value_dtype):
+    rng = np.random.default_rng(9)
+    n = 200  # indices fit every value width
+    keys = gen(rng, np.int32, n, "and2")
+    for k in (1, 50, 199):
+        check(keys, k, True, value_dtype=value_dtype)
+
+
+def test_sixteen_byte_values():
+    import cccl_amd as ca
+
+    rng = np.random.default_rng(10)
+    n, k = 50_000, 777
+    keys = gen(rng, np.float32, n)
+    vals = np.zeros((n, 4), np.uint32)
+    vals[:, 0] = np.arange(n)
+    vals[:, 3] = np.arange(n) * 3 + 1
+    kd, vd = to_dev(keys), torch.from_numpy(vals).to("cuda")
+    ko = torch.empty(k, dtype=kd.dtype, device="cuda")
+    vo = torch.empty((k, 4), dtype=vd.dtype, device="cuda")
+    ca.topk.max_pairs(kd, ko, vd, vo, n, k)
+    torch.cuda.synchronize()
+    v = vo.cpu().numpy()
+    idx = v[:, 0].astype(np.int64)
+    assert np.array_equal(v[:, 3], idx * 3 + 1)
+    assert np.array_equal(np.sort(oracle.topk_order_bits(keys[idx], True)), oracle.topk(keys, k, True))
+
+
+def test_selection_completes_early():
+    """Keys whose top digit alone decides the selection: the bins after the first pass
+    are skipped and the output compares the top digit only."""
+    n = 256 * 1000
+    keys = ((np.arange(n, dtype=np.uint32) % 256) << np.uint32(24)) | np.uint32(12345)
+    keys = np.random.default_rng(1).permutation(keys)
+    for k in (37 * 1000, 1000, 255 * 1000):
+        check(keys, k, True)
+        check(keys, k, False)
+
+
+def test_capped_k_zero_k_and_all_equal():
+    rng = np.random.default_rng(4)
+    keys = gen(rng, np.uint32, 5000)
+    check(keys, 5000 + 17, True)  # capped to num_items
+    same = np.full(10_000, 42, np.int64)
+    for k in (1, 5000, 9999):
+        check(same, k, False)
+    import cccl_amd as ca
+
+    kd = to_dev(keys)
+    ko = torch.full((4,), 7, dtype=kd.dtype, device="cuda")
+    ca.topk.max_keys(kd, ko, keys.size, 0)
+    ca.topk.max_keys(kd[:0], ko, 0, 3)
+    torch.cuda.synchronize()
+    assert (ko.cpu() == 7).all()
+
+
+def test_large_uniform():
+    rng = np.random.default_rng(12)
+    keys = gen(rng, np.uint32, 1 << 24)
+    for k in (1 << 10, 1 << 20):
+        check(keys, k, True)
+
+
+def test_two_phase_temp_storage():
+    import cccl_amd as ca
+
+    rng = np.random.default_rng(13)
+    keys = gen(rng, np.float64, 100_000)
+    kd = to_dev(keys)
+    ko = torch.empty(333, dtype=kd.dtype, device="cuda")
+    nbytes = ca.topk.min_keys(kd, ko, keys.size, 333, temp_storage=None)
+    assert nbytes == ca.topk.temp_storage_bytes(kd, keys.size, 333)
+    temp = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    for _ in range(2):  # the state in temp storage is reset by every call
+        ko.zero_()
+        ca.topk.min_keys(kd, ko, keys.size, 333, temp_storage=temp)
+        torch.cuda.synchronize()
+        got = np.sort(oracle.topk_order_bits(to_host(ko, keys.dtype), False))
+        assert np.array_equal(got, oracle.topk(keys, 333, False))
