@@ -21,6 +21,8 @@ upsweep, a bin scan and a downsweep).  Prints one JSON line with:
   size-binned batch, its hipGraph replay, CUB's one-workgroup-per-segment-per-pass
   schedule, and a per-segment loop of plain sorts (timed on a random sample of
   segments, scaled by the segment count);
+  the same batch through the cub::DeviceSegmentedSort C-ABI (cccl_device_segmented_sort);
+* cub::DeviceTopK::MaxPairs over 2^28 uint32 pairs (k = 2^10, 2^20) against the full sort;
 * the learned config selector on held-out calls of the 2,000-call stream, timed live
   against the fixed default configuration and the exhaustive best;
 * the reference's Thrust OpenMP stable_sort_by_key on a bounded sample, on this host.
@@ -343,6 +345,47 @@ def leg_segmented(torch, ca, dev, steps):
     }
 
 
+def leg_topk(torch, ca, dev, steps, sort_ms):
+    """cub::DeviceTopK::MaxPairs on 2^28 uniform uint32 keys + uint32 values (value =
+    index), k = 2^10 and 2^20, against the full sort of the same pairs (sort_ms).  The
+    algorithmic bytes of a top-k call are one read of the keys plus the k output pairs."""
+    n = 1 << 28
+    kin = gen_keys(torch, n, "uniform", 47, dev)
+    vin = torch.arange(n, dtype=torch.int32, device=dev).view(torch.uint32)
+    kall = kin.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    out = {"workload": f"MaxPairs over {n} uniform uint32 keys + uint32 values (value = index)"}
+    for k in (1 << 10, 1 << 20):
+        ko = torch.empty(k, dtype=torch.int32, device=dev).view(torch.uint32)
+        vo = torch.empty_like(ko)
+        nbytes = ca.topk.max_pairs(kin, ko, vin, vo, n, k, temp_storage=None)
+        temp = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+        fn = lambda: ca.topk.max_pairs(kin, ko, vin, vo, n, k, temp_storage=temp)  # noqa: E731
+        ms = time_steps(torch, fn, steps, 2)
+        launches = kernel_times(torch, fn).get(11, [])  # init, (histogram, pick) per digit, output
+        hist_ms = launches[1:-1:2]
+        # validity on the GPU: with t the smallest returned key, every key above t is
+        # returned, enough keys equal t, and each value indexes its own key
+        kk = ko.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+        thr = int(kk.min())
+        n_gt, n_ge = int((kall > thr).sum()), int((kall >= thr).sum())
+        v = vo.view(torch.int32).to(torch.int64)
+        ok = n_gt < k <= n_ge and int((kk > thr).sum()) == n_gt
+        ok &= bool((kin.view(torch.int32)[v] == ko.view(torch.int32)).all()) and torch.unique(v).numel() == k
+        alg = 4 * n + 8 * k
+        out[f"k_{k}"] = {
+            "ms": round(ms, 4), "gkeys_per_s": round(n / (ms * 1e-3) / 1e9, 2),
+            "alg_bytes": alg, "alg_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac_of_alg": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "histogram_ms_per_digit": [round(x, 4) for x in hist_ms],
+            "histogram_read_GBs_first_digit": round(4 * n / (hist_ms[0] * 1e-3) / 1e9, 1) if hist_ms else None,
+            "output_ms": round(launches[-1], 4) if launches else None,
+            "speedup_vs_full_sort": round(sort_ms / ms, 2), "valid": ok,
+        }
+        del ko, vo, temp
+    del kin, vin, kall
+    return out
+
+
 def leg_selector(torch, dev, calls: int):
     """Held-out calls of the 2,000-call stream (the last 30%, every k-th): the shipped
     network picks an arm from features computed live; the pick, the fixed default arm
@@ -500,6 +543,8 @@ def main() -> int:
         extra["wide_keys"] = leg_wide_keys(torch, ca, dev, max(2, args.steps // 2))
         torch.cuda.empty_cache()
         extra["segmented_batch"] = leg_segmented(torch, ca, dev, max(2, args.steps // 2))
+        torch.cuda.empty_cache()
+        extra["device_topk"] = leg_topk(torch, ca, dev, max(2, args.steps // 2), ms)
         torch.cuda.empty_cache()
         if args.selector_calls > 0:
             extra["learned_selector"] = leg_selector(torch, dev, args.selector_calls)

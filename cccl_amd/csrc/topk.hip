@@ -18,7 +18,9 @@
 //                  prefix |= d, k_rem -= candidates below d; when the bin is taken whole,
 //                  the selection is complete and later passes return at once
 //   topk_output  keys with o below the threshold take slots [0, k - k_rem) and keys equal
-//                to it the first k_rem slots after that, both by wave-aggregated atomics.
+//                to it the first k_rem slots after that: each workgroup stages the items
+//                it selects in LDS and flushes a batch with one global atomic (its base
+//                slot) and coalesced stores.
 // Each histogram pass reads every key once (K bytes per item), so a call moves at most
 // (P + 1) * N * K + k * (K + V) bytes for P digit passes.
 #include <algorithm>
@@ -175,16 +177,16 @@ __device__ __forceinline__ void copy_value(void* out, const void* in, uint64_t d
   }
 }
 
-// Adds this wave's popcount(m) to *counter (one atomic, issued by the first lane in m)
-// and returns the slot of this lane among the lanes in m.
-__device__ __forceinline__ uint32_t wave_slot(uint32_t* counter, uint64_t m)
+// Slot of this lane among the lanes of m, after adding popcount(m) to the LDS counter
+// *n with one atomic per wave (issued by the first lane in m).
+__device__ __forceinline__ uint32_t wave_lds_slot(uint32_t* n, uint64_t m)
 {
   const int leader = __builtin_ctzll(m);
   uint32_t base    = 0;
   if (int(lane_id()) == leader)
   {
-    base = __hip_atomic_fetch_add(counter, uint32_t(__builtin_popcountll(m)), __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
+    base = __hip_atomic_fetch_add(n, uint32_t(__builtin_popcountll(m)), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   base = __builtin_amdgcn_readlane(base, leader);
   return base + mask_prefix_count(m);
@@ -203,54 +205,149 @@ struct TopkOutputParams
   int select_max;
 };
 
+// LDS staging of the output: each workgroup collects the items it selects in two LDS
+// buffers of CAP items (below / equal to the threshold) and flushes a buffer -- one
+// global atomic for its base slot, then coalesced stores -- once it holds more than
+// CAP / 2 items, so one pass of CAP / 2 items always fits.  CAP keeps both buffers of
+// keys and values within 32 KiB (four workgroups per CU).
+template <class UK, class V>
+struct TopkStage
+{
+  static constexpr bool kHasValue = !std::is_same<V, NoValue>::value;
+  static constexpr int kItemBytes = int(sizeof(UK)) + (kHasValue ? int(sizeof(V)) : 0);
+  static constexpr int kCap       = kItemBytes <= 8 ? 2048 : (kItemBytes <= 16 ? 1024 : 512);
+  static constexpr int kIpt       = kCap / (2 * kTopkThreads); // one pass: kCap / 2 items
+  static constexpr int kValCap    = kHasValue ? kCap : 1;
+  using VS                        = typename std::conditional<kHasValue, V, uint8_t>::type;
+};
+
+// Writes the buffered items [0, count) to the output from slot `base` on, keeping only
+// slots below `limit` (relative to `base`'s origin); every thread of the workgroup calls it.
+template <class UK, class V>
+__device__ __forceinline__ void topk_flush(const UK* s_keys, const typename TopkStage<UK, V>::VS* s_vals,
+                                           uint32_t count, uint64_t base, uint64_t limit, uint64_t out0,
+                                           UK* keys_out, void* vals_out)
+{
+  for (uint32_t i = threadIdx.x; i < count; i += kTopkThreads)
+  {
+    if (base + i < limit)
+    {
+      keys_out[out0 + base + i] = s_keys[i];
+      if constexpr (TopkStage<UK, V>::kHasValue)
+      {
+        static_cast<V*>(vals_out)[out0 + base + i] = s_vals[i];
+      }
+    }
+  }
+}
+
 template <class UK, class V>
 __global__ void __launch_bounds__(kTopkThreads) topk_output_kernel(TopkOutputParams p)
 {
+  using S                  = TopkStage<UK, V>;
+  constexpr int kCap       = S::kCap;
+  constexpr int kIpt       = S::kIpt;
+  constexpr uint32_t kFull = uint32_t(kCap / 2); // flush above this many items
+  __shared__ UK s_klt[kCap];
+  __shared__ UK s_keq[kCap];
+  __shared__ typename S::VS s_vlt[S::kValCap];
+  __shared__ typename S::VS s_veq[S::kValCap];
+  __shared__ uint32_t s_n[2]; // items buffered: below, equal
+  __shared__ uint64_t s_base;
+  __shared__ uint32_t s_eq_full;
+
   TopkState* st         = p.st;
   const int s           = st->done != 0u ? int(st->done_shift) : 0;
   const UK thr          = UK(UK(st->prefix) >> s);
   const uint64_t k_rem  = st->k_rem;
-  const uint64_t eq0    = p.k - k_rem; // first slot of the keys equal to the threshold
+  const uint64_t eq0    = p.k - k_rem; // keys below the threshold fill [0, eq0), equal ones [eq0, k)
   const UK* keys        = static_cast<const UK*>(p.keys_in);
   UK* keys_out          = static_cast<UK*>(p.keys_out);
   const bool select_max = p.select_max != 0;
-  const uint64_t chunk  = uint64_t(kTopkThreads) * kTopkIpt;
+  if (threadIdx.x == 0)
+  {
+    s_n[0] = s_n[1] = 0u;
+    s_eq_full       = 0u;
+  }
+  __syncthreads();
+
+  const uint64_t chunk = uint64_t(kTopkThreads) * kIpt;
   for (uint64_t base = uint64_t(blockIdx.x) * chunk; base < p.num_items; base += uint64_t(gridDim.x) * chunk)
   {
-    UK k[kTopkIpt];
+    UK k[kIpt];
 #pragma unroll
-    for (int j = 0; j < kTopkIpt; ++j)
+    for (int j = 0; j < kIpt; ++j)
     {
       const uint64_t i = base + uint64_t(j) * kTopkThreads + threadIdx.x;
       k[j]             = i < p.num_items ? keys[i] : UK(0);
     }
+    const bool take_eq = s_eq_full == 0u;
 #pragma unroll
-    for (int j = 0; j < kTopkIpt; ++j)
+    for (int j = 0; j < kIpt; ++j)
     {
       const uint64_t i   = base + uint64_t(j) * kTopkThreads + threadIdx.x;
       const bool in      = i < p.num_items;
       const UK o         = UK(topk_order(k[j], p.key_kind, select_max) >> s);
-      const uint64_t mlt = __ballot(in && o < thr);
-      const uint64_t meq = __ballot(in && o == thr);
-      if (mlt != 0u && in && o < thr)
+      const bool lt      = in && o < thr;
+      const bool eq      = in && take_eq && o == thr;
+      const uint64_t mlt = __ballot(lt);
+      const uint64_t meq = __ballot(eq);
+      if (lt)
       {
-        const uint64_t slot = wave_slot(&st->count_lt, mlt);
-        if (slot < eq0) // always: exactly eq0 keys lie below the threshold
+        const uint32_t slot = wave_lds_slot(&s_n[0], mlt);
+        s_klt[slot]         = k[j];
+        if constexpr (S::kHasValue)
         {
-          keys_out[slot] = k[j];
-          copy_value<V>(p.vals_out, p.vals_in, slot, i);
+          s_vlt[slot] = static_cast<const V*>(p.vals_in)[i];
         }
       }
-      if (meq != 0u && in && o == thr)
+      if (eq)
       {
-        const uint64_t slot = wave_slot(&st->count_eq, meq);
-        if (slot < k_rem)
+        const uint32_t slot = wave_lds_slot(&s_n[1], meq);
+        s_keq[slot]         = k[j];
+        if constexpr (S::kHasValue)
         {
-          keys_out[eq0 + slot] = k[j];
-          copy_value<V>(p.vals_out, p.vals_in, eq0 + slot, i);
+          s_veq[slot] = static_cast<const V*>(p.vals_in)[i];
         }
       }
     }
+    __syncthreads();
+    const bool last = base + uint64_t(gridDim.x) * chunk >= p.num_items;
+    const uint32_t nlt = s_n[0];
+    const uint32_t neq = s_n[1];
+    if (nlt > kFull || (last && nlt > 0u))
+    {
+      if (threadIdx.x == 0)
+      {
+        s_base = __hip_atomic_fetch_add(&st->count_lt, nlt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __syncthreads();
+      topk_flush<UK, V>(s_klt, s_vlt, nlt, s_base, eq0, 0, keys_out, p.vals_out); // below eq0: always
+      __syncthreads();
+      if (threadIdx.x == 0)
+      {
+        s_n[0] = 0u;
+      }
+    }
+    if (neq > kFull || (last && neq > 0u))
+    {
+      if (threadIdx.x == 0)
+      {
+        s_base = __hip_atomic_fetch_add(&st->count_eq, neq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s_base + neq >= k_rem)
+        {
+          s_eq_full = 1u; // the tied slots are all taken: stop collecting equal keys
+        }
+      }
+      __syncthreads();
+      topk_flush<UK, V>(s_keq, s_veq, neq, s_base, k_rem, eq0, keys_out, p.vals_out);
+      __syncthreads();
+      if (threadIdx.x == 0)
+      {
+        s_n[1] = 0u;
+      }
+    }
+    __syncthreads();
   }
 }
 
