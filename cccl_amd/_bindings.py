@@ -194,6 +194,7 @@ EXPORTED_SYMBOLS = [
     "cccl_device_segmented_sort_cleanup",
     "cccl_amd_radix_sort",
     "cccl_amd_segmented_radix_sort",
+    "cccl_amd_segmented_sort",
     "cccl_amd_radix_sort_default_config",
     "cccl_amd_segmented_default_config",
     "cccl_amd_radix_sort_config_valid",
@@ -276,6 +277,9 @@ def lib():
                                                 sz, u64, i64, vp, vp, i32, i32, i32, i32, i32,
                                                 ctypes.POINTER(segmented_config), vp]
     L.cccl_amd_segmented_radix_sort.restype = i32
+    L.cccl_amd_segmented_sort.argtypes = [vp, ctypes.POINTER(sz), vp, vp, vp, vp, ctypes.POINTER(i32), i32, sz, u64,
+                                          i64, vp, vp, i32, i32, i32, ctypes.POINTER(segmented_config), vp]
+    L.cccl_amd_segmented_sort.restype = i32
     L.cccl_amd_radix_sort_default_config.argtypes = [i32, i32, u64, ctypes.POINTER(radix_sort_config)]
     L.cccl_amd_radix_sort_default_config.restype = None
     L.cccl_amd_segmented_default_config.argtypes = [i32, i32, ctypes.POINTER(segmented_config)]

@@ -81,6 +81,27 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     return LIB
 
 
+CPP_TESTS = [REPO / "tests" / "cpp" / "test_device_api.cpp"]
+
+
+def build_cpp_tests(verbose: bool = False) -> list[Path]:
+    """Host programs that drive the C++ API (include/cccl_amd/device_radix_sort.hpp)
+    against libcccl_amd.so; each binary sits beside its source."""
+    out = []
+    for src in CPP_TESTS:
+        exe = src.with_suffix("")
+        cmd = [_hipcc(), *HIP_FLAGS, "-I", str(REPO / "include"), str(src), "-L", str(PKG), "-lcccl_amd",
+               "-Wl,-rpath,$ORIGIN/../../cccl_amd", "-o", str(exe)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stdout}\n{r.stderr}")
+        out.append(exe)
+        if verbose:
+            print(f"built {exe}")
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_cpp_tests(verbose=True)
     sys.exit(0)

@@ -656,6 +656,74 @@ CCCL_C_API int cccl_amd_segmented_radix_sort(
   return int(e);
 }
 
+CCCL_C_API int cccl_amd_segmented_sort(
+  void* d_temp_storage,
+  size_t* temp_storage_bytes,
+  void* keys0,
+  void* keys1,
+  void* vals0,
+  void* vals1,
+  int* selector,
+  cccl_type_enum key_type,
+  size_t value_bytes,
+  uint64_t num_items,
+  int64_t num_segments,
+  const void* d_begin_offsets,
+  const void* d_end_offsets,
+  int offset_bytes,
+  int descending,
+  int is_overwrite_okay,
+  const cccl_amd_segmented_config_t* cfg,
+  CUstream stream)
+{
+  if (temp_storage_bytes == nullptr || selector == nullptr || (*selector != 0 && *selector != 1) || num_segments < 0
+      || (offset_bytes != 4 && offset_bytes != 8))
+  {
+    return hipErrorInvalidValue;
+  }
+  SortProblem pr{};
+  if (!key_traits(key_type, pr.key_bytes, pr.key_kind) || !value_size_ok(value_bytes))
+  {
+    return hipErrorNotSupported;
+  }
+  if (value_bytes > 0 && d_temp_storage != nullptr && (vals0 == nullptr || vals1 == nullptr))
+  {
+    cccl_amd::set_last_error("value_bytes is %zu but a value buffer is null", value_bytes);
+    return hipErrorInvalidValue;
+  }
+  pr.val_bytes  = int(value_bytes);
+  pr.num_items  = num_items;
+  pr.begin_bit  = 0;
+  pr.end_bit    = pr.key_bytes * 8;
+  pr.descending = descending != 0;
+  pr.overwrite  = is_overwrite_okay != 0;
+  cccl_amd_segmented_config_t c{};
+  if (cfg != nullptr)
+  {
+    c = *cfg;
+  }
+  else
+  {
+    cccl_amd_segmented_default_config(pr.key_bytes, pr.val_bytes, &c);
+  }
+  DoubleBufferRef keys{{keys0, keys1}, *selector};
+  DoubleBufferRef vals{{vals0, vals1}, *selector};
+  const hipError_t e = cccl_amd::segmented_sort(
+    d_temp_storage,
+    *temp_storage_bytes,
+    keys,
+    vals,
+    pr,
+    num_segments,
+    d_begin_offsets,
+    d_end_offsets,
+    offset_bytes,
+    c,
+    reinterpret_cast<hipStream_t>(stream));
+  *selector = keys.selector;
+  return int(e);
+}
+
 CCCL_C_API void cccl_amd_radix_sort_default_config(
   int key_bytes, int value_bytes, uint64_t num_items, cccl_amd_radix_sort_config_t* cfg)
 {

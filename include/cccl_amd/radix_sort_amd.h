@@ -1,6 +1,6 @@
 /* Typed C entry points of the MI355X radix sort, below the cccl_device_radix_sort_*
- * C-ABI.  The C++ API (cccl_amd/device_radix_sort.hpp) and the Python binding call
- * these; they carry the launch configuration explicitly so the autotuner and the
+ * C-ABI.  The C++ API (include/cccl_amd/device_radix_sort.hpp) and the Python binding
+ * call these; they carry the launch configuration explicitly so the autotuner and the
  * config selector can pick it per call.
  *
  * Return values are hipError_t codes (0 = success, 1 = invalid value, 2 = out of
@@ -116,6 +116,35 @@ CCCL_C_API int cccl_amd_segmented_radix_sort(
   int offset_bytes,
   int begin_bit,
   int end_bit,
+  int descending,
+  int is_overwrite_okay,
+  const cccl_amd_segmented_config_t* cfg,
+  CUstream stream);
+
+/* cub::DeviceSegmentedSort over the full key (cub/cub/device/device_segmented_sort.cuh;
+ * dispatch cub/cub/device/dispatch/dispatch_segmented_sort.cuh:1266-1541): every segment
+ * sorted whole, equal keys in input order (so it serves SortKeys / SortPairs and their
+ * Stable* forms alike).  Buffer and selector rules are DeviceSegmentedSort's: without
+ * is_overwrite_okay the result is in the alternate buffer (keys1 / vals1 and *selector = 1
+ * for the pointer form, *selector = 0 on entry); with it the result is
+ * in buffer (*selector + P) & 1, P = ceil(8K / 6) for K-byte keys with K > 1 and 2 for
+ * one-byte keys (dispatch_segmented_sort.cuh:1293-1299); *selector is left unchanged when
+ * num_items or num_segments is 0.  Offsets as cccl_amd_segmented_radix_sort. */
+CCCL_C_API int cccl_amd_segmented_sort(
+  void* d_temp_storage,
+  size_t* temp_storage_bytes,
+  void* keys0,
+  void* keys1,
+  void* vals0,
+  void* vals1,
+  int* selector,
+  cccl_type_enum key_type,
+  size_t value_bytes,
+  uint64_t num_items,
+  int64_t num_segments,
+  const void* d_begin_offsets,
+  const void* d_end_offsets,
+  int offset_bytes,
   int descending,
   int is_overwrite_okay,
   const cccl_amd_segmented_config_t* cfg,
