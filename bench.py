@@ -379,7 +379,7 @@ def leg_topk(torch, ca, dev, steps, sort_ms):
             "histogram_ms_per_digit": [round(x, 4) for x in hist_ms],
             "histogram_read_GBs_first_digit": round(4 * n / (hist_ms[0] * 1e-3) / 1e9, 1) if hist_ms else None,
             "output_ms": round(launches[-1], 4) if launches else None,
-            "speedup_vs_full_sort": round(sort_ms / ms, 2), "valid": ok,
+            "speedup_vs_full_sort": round(sort_ms / ms, 2) if sort_ms else None, "valid": ok,
         }
         del ko, vo, temp
     del kin, vin, kall
@@ -430,6 +430,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample-log2", type=int, default=25)
     ap.add_argument("--segmented-profile", action="store_true",
                     help="only the segmented batch with the default configuration (for a kernel trace)")
+    ap.add_argument("--topk-profile", action="store_true",
+                    help="only the DeviceTopK MaxPairs leg (for a kernel trace of its launches)")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="counter runs: one 2 GiB device copy (reference bytes), then --steps sorts only")
     args = ap.parse_args()
@@ -446,6 +448,9 @@ def main() -> int:
     dev = torch.device("cuda:0")
     if args.segmented_profile:
         print(json.dumps(segmented_profile(torch, dev, args.steps)), flush=True)
+        return 0
+    if args.topk_profile:
+        print(json.dumps(leg_topk(torch, ca, dev, args.steps, None)), flush=True)
         return 0
     n = 1 << args.log2n
     entropies = [e for e in args.entropies.split(",") if e]

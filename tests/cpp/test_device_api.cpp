@@ -19,6 +19,7 @@
 #include <vector>
 
 #include <cccl_amd/device_radix_sort.hpp>
+#include <cccl_amd/thrust_sort.hpp>
 
 namespace {
 
@@ -337,6 +338,53 @@ void topk_case()
   report("DeviceTopK::MaxPairs i32", ok);
 }
 
+// thrust::sort / stable_sort_by_key through the radix route (in place, copy back when the
+// result ends in the temporary half: one-byte keys take one pass).
+template <typename K, typename Compare>
+void thrust_keys_case(const char* name, size_t n, uint64_t seed, Compare comp)
+{
+  auto keys = random_keys<K>(n, seed);
+  Dev<K> d(keys);
+  HIP_CHECK(cccl_amd::thrust_compat::sort(d.p, d.p + n, comp));
+  HIP_CHECK(hipDeviceSynchronize());
+  std::vector<uint32_t> perm(n);
+  std::iota(perm.begin(), perm.end(), 0u);
+  stable_perm(keys, 0, n, std::is_same_v<Compare, std::greater<K>> || std::is_same_v<Compare, std::greater<>>, 0,
+              int(sizeof(K) * 8), perm);
+  std::vector<K> ek(n);
+  for (size_t i = 0; i < n; ++i)
+  {
+    ek[i] = keys[perm[i]];
+  }
+  report(name, same_bits(d.host(n), ek));
+}
+
+void thrust_pairs_case()
+{
+  const size_t n = 250000;
+  auto keys      = random_keys<double>(n, 21);
+  std::vector<uint64_t> vals(n);
+  std::iota(vals.begin(), vals.end(), uint64_t(0));
+  Dev<double> dk(keys);
+  Dev<uint64_t> dv(vals);
+  hipStream_t s = nullptr;
+  HIP_CHECK(hipStreamCreate(&s));
+  HIP_CHECK(cccl_amd::thrust_compat::stable_sort_by_key(dk.p, dk.p + n, dv.p, std::greater<double>(), s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  HIP_CHECK(hipStreamDestroy(s));
+  std::vector<uint32_t> perm(n);
+  std::iota(perm.begin(), perm.end(), 0u);
+  stable_perm(keys, 0, n, true, 0, 64, perm);
+  std::vector<double> ek(n);
+  std::vector<uint64_t> ev(n);
+  for (size_t i = 0; i < n; ++i)
+  {
+    ek[i] = keys[perm[i]];
+    ev[i] = perm[i];
+  }
+  report("thrust_compat::stable_sort_by_key f64 greater, own stream", same_bits(dk.host(n), ek) && dv.host(n) == ev);
+}
+
 bool query_only()
 {
   size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
@@ -371,6 +419,9 @@ int main(int argc, char** argv)
                                       true, 250, 15);
     segmented_case<float, int32_t>("DeviceSegmentedSort::StableSortPairs f32 DoubleBuffer", true, false, true, 250, 16);
     topk_case();
+    thrust_keys_case<uint32_t>("thrust_compat::sort u32 less", (1 << 20) + 5, 22, std::less<uint32_t>());
+    thrust_keys_case<uint8_t>("thrust_compat::sort u8 greater<> (copy back)", 100000, 23, std::greater<>());
+    thrust_pairs_case();
   }
   std::printf("%s: %d failure(s)\n", g_failures ? "FAILED" : "OK", g_failures);
   return g_failures ? 1 : 0;

@@ -1,5 +1,6 @@
 """The C++ API (include/cccl_amd/device_radix_sort.hpp: DeviceRadixSort,
-DeviceSegmentedRadixSort, DeviceSegmentedSort, DeviceTopK, DoubleBuffer) through the host
+DeviceSegmentedRadixSort, DeviceSegmentedSort, DeviceTopK, DoubleBuffer; and
+include/cccl_amd/thrust_sort.hpp: Thrust's radix route) through the host
 program tests/cpp/test_device_api, which checks every case against std::stable_sort on
 the host.  build() compiles the program beside its source."""
 
@@ -29,4 +30,4 @@ def test_cpp_api_size_queries_without_a_device():
 def test_cpp_api_on_the_gpu():
     r = _run()
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("PASS") == 11 and "FAIL" not in r.stdout
+    assert r.stdout.count("PASS") == 14 and "FAIL" not in r.stdout
