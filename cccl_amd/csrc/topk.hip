@@ -13,7 +13,8 @@
 //   topk_init    the selection state: prefix 0, k_rem = k, counters and bins 0
 //   per 8-bit digit, most significant first:
 //     topk_hist    bins of this digit over the keys whose higher digits equal the prefix
-//                  (LDS bins per workgroup, one global atomic per non-zero bin)
+//                  (LDS bins per workgroup, one global atomic per non-zero bin); the
+//                  second digit's pass is topk_filter instead (below)
 //     topk_pick    one workgroup: the digit d whose bin holds the k_rem-th candidate;
 //                  prefix |= d, k_rem -= candidates below d; when the bin is taken whole,
 //                  the selection is complete and later passes return at once
@@ -21,8 +22,12 @@
 //                to it the first k_rem slots after that: each workgroup stages the items
 //                it selects in LDS and flushes a batch with one global atomic (its base
 //                slot) and coalesced stores.
-// Each histogram pass reads every key once (K bytes per item), so a call moves at most
-// (P + 1) * N * K + k * (K + V) bytes for P digit passes.
+// topk_filter (the second digit pass) also writes the keys below the top digit straight to
+// the output and the keys on it -- the candidates -- with their input indices to a
+// buffer of up to N / 16 items; the later passes and topk_output read that buffer instead
+// of the input when the candidates fit (for spread keys about N / 256 of them), so only
+// the first two passes read every key: about 2 * N * K bytes plus the candidates and the
+// k outputs, against (P + 1) * N * K when every pass reads the input.
 #include <algorithm>
 
 #include "../../include/cccl_amd/topk.h"
@@ -44,6 +49,8 @@ struct TopkState
   uint32_t done_shift;
   uint32_t count_lt; // output slots taken by keys below the threshold
   uint32_t count_eq; // output slots taken by keys equal to the threshold
+  uint32_t filtered; // 1: the filter pass ran (keys below the top digit are out already)
+  uint32_t cand_count; // candidates the filter pass found (kept when at most cand_cap)
   uint32_t bins[kRadix];
 };
 
@@ -80,13 +87,37 @@ __global__ void __launch_bounds__(kTopkThreads) topk_init_kernel(TopkInitParams 
     p.st->done_shift = 0u;
     p.st->count_lt   = 0u;
     p.st->count_eq   = 0u;
+    p.st->filtered   = 0u;
+    p.st->cand_count = 0u;
   }
+}
+
+// Where a pass reads its keys: the whole input, or -- once the filter pass has run and
+// its candidates fit -- the candidate buffer (keys and their input indices).
+struct TopkSource
+{
+  const void* keys;
+  const void* cand_keys;
+  const uint32_t* cand_idx;
+  uint64_t num_items;
+  uint32_t cand_cap;
+};
+
+struct TopkSourceView
+{
+  bool cand;
+  uint64_t count;
+};
+
+__device__ __forceinline__ TopkSourceView topk_source(const TopkState* st, const TopkSource& src)
+{
+  const bool cand = st->filtered != 0u && st->cand_count <= src.cand_cap;
+  return {cand, cand ? uint64_t(st->cand_count) : src.num_items};
 }
 
 struct TopkPassParams
 {
-  const void* keys;
-  uint64_t num_items;
+  TopkSource src;
   TopkState* st;
   int key_kind;
   int select_max;
@@ -101,29 +132,30 @@ __global__ void __launch_bounds__(kTopkThreads) topk_hist_kernel(TopkPassParams 
   {
     return;
   }
-  constexpr int kBits   = int(sizeof(UK) * 8);
-  const int hi          = p.shift + kRadixBits;
-  const UK prefix_hi    = hi >= kBits ? UK(0) : UK(UK(p.st->prefix) >> hi);
-  const UK* keys        = static_cast<const UK*>(p.keys);
-  const bool select_max = p.select_max != 0;
-  bins[threadIdx.x]     = 0u;
+  constexpr int kBits       = int(sizeof(UK) * 8);
+  const int hi              = p.shift + kRadixBits;
+  const UK prefix_hi        = hi >= kBits ? UK(0) : UK(UK(p.st->prefix) >> hi);
+  const TopkSourceView view = topk_source(p.st, p.src);
+  const UK* keys            = static_cast<const UK*>(view.cand ? p.src.cand_keys : p.src.keys);
+  const bool select_max     = p.select_max != 0;
+  bins[threadIdx.x]         = 0u;
   __syncthreads();
   const uint64_t chunk = uint64_t(kTopkThreads) * kTopkIpt;
-  for (uint64_t base = uint64_t(blockIdx.x) * chunk; base < p.num_items; base += uint64_t(gridDim.x) * chunk)
+  for (uint64_t base = uint64_t(blockIdx.x) * chunk; base < view.count; base += uint64_t(gridDim.x) * chunk)
   {
     UK k[kTopkIpt];
 #pragma unroll
     for (int j = 0; j < kTopkIpt; ++j)
     {
       const uint64_t i = base + uint64_t(j) * kTopkThreads + threadIdx.x;
-      k[j]             = i < p.num_items ? keys[i] : UK(0);
+      k[j]             = i < view.count ? keys[i] : UK(0);
     }
 #pragma unroll
     for (int j = 0; j < kTopkIpt; ++j)
     {
       const uint64_t i = base + uint64_t(j) * kTopkThreads + threadIdx.x;
       const UK o       = topk_order(k[j], p.key_kind, select_max);
-      const bool match = i < p.num_items && (hi >= kBits || UK(o >> hi) == prefix_hi);
+      const bool match = i < view.count && (hi >= kBits || UK(o >> hi) == prefix_hi);
       if (match)
       {
         lds_count(bins, extract_digit(o, p.shift, kRadix - 1));
@@ -168,15 +200,6 @@ __global__ void __launch_bounds__(kTopkThreads) topk_pick_kernel(TopkPassParams 
   }
 }
 
-template <class V>
-__device__ __forceinline__ void copy_value(void* out, const void* in, uint64_t dst, uint64_t src)
-{
-  if constexpr (!std::is_same<V, NoValue>::value)
-  {
-    static_cast<V*>(out)[dst] = static_cast<const V*>(in)[src];
-  }
-}
-
 // Slot of this lane among the lanes of m, after adding popcount(m) to the LDS counter
 // *n with one atomic per wave (issued by the first lane in m).
 __device__ __forceinline__ uint32_t wave_lds_slot(uint32_t* n, uint64_t m)
@@ -192,37 +215,23 @@ __device__ __forceinline__ uint32_t wave_lds_slot(uint32_t* n, uint64_t m)
   return base + mask_prefix_count(m);
 }
 
-struct TopkOutputParams
-{
-  const void* keys_in;
-  const void* vals_in;
-  void* keys_out;
-  void* vals_out;
-  uint64_t num_items;
-  uint64_t k;
-  TopkState* st;
-  int key_kind;
-  int select_max;
-};
-
-// LDS staging of the output: each workgroup collects the items it selects in two LDS
-// buffers of CAP items (below / equal to the threshold) and flushes a buffer -- one
-// global atomic for its base slot, then coalesced stores -- once it holds more than
-// CAP / 2 items, so one pass of CAP / 2 items always fits.  CAP keeps both buffers of
-// keys and values within 32 KiB (four workgroups per CU).
+// LDS staging of selected items: a workgroup collects the items it selects in LDS
+// buffers of CAP items and flushes a buffer -- one global atomic for its base slot, then
+// coalesced stores -- once it holds more than CAP / 2 items, so one loop step of CAP / 2
+// items always fits.  CAP keeps two buffers of keys and values within 32 KiB.
 template <class UK, class V>
 struct TopkStage
 {
   static constexpr bool kHasValue = !std::is_same<V, NoValue>::value;
   static constexpr int kItemBytes = int(sizeof(UK)) + (kHasValue ? int(sizeof(V)) : 0);
   static constexpr int kCap       = kItemBytes <= 8 ? 2048 : (kItemBytes <= 16 ? 1024 : 512);
-  static constexpr int kIpt       = kCap / (2 * kTopkThreads); // one pass: kCap / 2 items
+  static constexpr int kIpt       = kCap / (2 * kTopkThreads); // one loop step: kCap / 2 items
   static constexpr int kValCap    = kHasValue ? kCap : 1;
   using VS                        = typename std::conditional<kHasValue, V, uint8_t>::type;
 };
 
-// Writes the buffered items [0, count) to the output from slot `base` on, keeping only
-// slots below `limit` (relative to `base`'s origin); every thread of the workgroup calls it.
+// Writes the buffered items [0, count) from slot `base` on (slots below `limit` only,
+// counted from `out0`); every thread of the workgroup calls it.
 template <class UK, class V>
 __device__ __forceinline__ void topk_flush(const UK* s_keys, const typename TopkStage<UK, V>::VS* s_vals,
                                            uint32_t count, uint64_t base, uint64_t limit, uint64_t out0,
@@ -241,6 +250,175 @@ __device__ __forceinline__ void topk_flush(const UK* s_keys, const typename Topk
   }
 }
 
+struct TopkFilterParams
+{
+  TopkSource src;
+  const void* vals_in;
+  void* keys_out;
+  void* vals_out;
+  uint64_t k;
+  TopkState* st;
+  void* cand_keys; // writable views of src.cand_keys / src.cand_idx
+  uint32_t* cand_idx;
+  int key_kind;
+  int select_max;
+};
+
+// The second digit pass, run once the top digit d0 is picked: over the whole input, keys
+// whose top digit is below d0 are selected for certain and go straight to the output;
+// keys whose top digit is d0 are the candidates: their second digit is counted, and they
+// (with their input index) go to the candidate buffer, which the later passes and the
+// output pass read instead of the input when the candidates fit.
+template <class UK, class V>
+__global__ void __launch_bounds__(kTopkThreads) topk_filter_kernel(TopkFilterParams p)
+{
+  using S                  = TopkStage<UK, V>;
+  constexpr int kCap       = S::kCap;
+  constexpr int kIpt       = S::kIpt;
+  constexpr uint32_t kFull = uint32_t(kCap / 2);
+  constexpr int kBits      = int(sizeof(UK) * 8);
+  constexpr int kTop       = kBits - kRadixBits;
+  __shared__ uint32_t bins[kRadix];
+  __shared__ UK s_klt[kCap];
+  __shared__ typename S::VS s_vlt[S::kValCap];
+  __shared__ UK s_kc[kCap];
+  __shared__ uint32_t s_ic[kCap];
+  __shared__ uint32_t s_n[2]; // items buffered: selected, candidates
+  __shared__ uint64_t s_base;
+
+  TopkState* st = p.st;
+  if (st->done != 0u)
+  {
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+  {
+    st->filtered = 1u;
+  }
+  const UK d0           = UK(UK(st->prefix) >> kTop);
+  const UK* keys        = static_cast<const UK*>(p.src.keys);
+  UK* keys_out          = static_cast<UK*>(p.keys_out);
+  UK* cand_keys         = static_cast<UK*>(p.cand_keys);
+  const bool select_max = p.select_max != 0;
+  const uint64_t n      = p.src.num_items;
+  bins[threadIdx.x]     = 0u;
+  if (threadIdx.x == 0)
+  {
+    s_n[0] = s_n[1] = 0u;
+  }
+  __syncthreads();
+
+  // Each step loads kTopkIpt keys per thread (all in flight together), then stages them
+  // in groups of kIpt with a flush check after each group, so a group always fits.
+  constexpr int kGroups = kTopkIpt / kIpt;
+  const uint64_t chunk  = uint64_t(kTopkThreads) * kTopkIpt;
+  for (uint64_t base = uint64_t(blockIdx.x) * chunk; base < n; base += uint64_t(gridDim.x) * chunk)
+  {
+    UK k[kTopkIpt];
+#pragma unroll
+    for (int j = 0; j < kTopkIpt; ++j)
+    {
+      const uint64_t i = base + uint64_t(j) * kTopkThreads + threadIdx.x;
+      k[j]             = i < n ? keys[i] : UK(0);
+    }
+    const bool last_step = base + uint64_t(gridDim.x) * chunk >= n;
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g)
+    {
+#pragma unroll
+      for (int jj = 0; jj < kIpt; ++jj)
+      {
+        const int j          = g * kIpt + jj;
+        const uint64_t i     = base + uint64_t(j) * kTopkThreads + threadIdx.x;
+        const bool in        = i < n;
+        const UK o           = topk_order(k[j], p.key_kind, select_max);
+        const UK top         = UK(o >> kTop);
+        const bool lt        = in && top < d0;
+        const bool cand      = in && top == d0;
+        const uint64_t mlt   = __ballot(lt);
+        const uint64_t mcnd  = __ballot(cand);
+        if (lt)
+        {
+          const uint32_t slot = wave_lds_slot(&s_n[0], mlt);
+          s_klt[slot]         = k[j];
+          if constexpr (S::kHasValue)
+          {
+            s_vlt[slot] = static_cast<const V*>(p.vals_in)[i];
+          }
+        }
+        if (cand)
+        {
+          lds_count(bins, extract_digit(o, kTop - kRadixBits, kRadix - 1));
+          const uint32_t slot = wave_lds_slot(&s_n[1], mcnd);
+          s_kc[slot]          = k[j];
+          s_ic[slot]          = uint32_t(i);
+        }
+      }
+      __syncthreads();
+      const bool last    = last_step && g == kGroups - 1;
+      const uint32_t nlt = s_n[0];
+      const uint32_t nc  = s_n[1];
+      if (nlt > kFull || (last && nlt > 0u))
+      {
+        if (threadIdx.x == 0)
+        {
+          s_base = __hip_atomic_fetch_add(&st->count_lt, nlt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        topk_flush<UK, V>(s_klt, s_vlt, nlt, s_base, p.k, 0, keys_out, p.vals_out); // always below k
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+          s_n[0] = 0u;
+        }
+      }
+      if (nc > kFull || (last && nc > 0u))
+      {
+        if (threadIdx.x == 0)
+        {
+          s_base = __hip_atomic_fetch_add(&st->cand_count, nc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < nc; c += kTopkThreads)
+        {
+          if (s_base + c < p.src.cand_cap) // beyond: the later passes read the input instead
+          {
+            cand_keys[s_base + c]  = s_kc[c];
+            p.cand_idx[s_base + c] = s_ic[c];
+          }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+          s_n[1] = 0u;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const uint32_t c = bins[threadIdx.x];
+  if (c != 0u)
+  {
+    __hip_atomic_fetch_add(&st->bins[threadIdx.x], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+struct TopkOutputParams
+{
+  TopkSource src;
+  const void* vals_in;
+  void* keys_out;
+  void* vals_out;
+  uint64_t k;
+  TopkState* st;
+  int key_kind;
+  int select_max;
+};
+
+// The last pass: keys below the threshold fill slots [count_lt, k - k_rem) (the filter
+// pass filled the first ones), keys equal to it the k_rem slots after that, both staged
+// in LDS per workgroup.  After a filter pass only keys with the top digit d0 are
+// considered (the others are out already or not selected).
 template <class UK, class V>
 __global__ void __launch_bounds__(kTopkThreads) topk_output_kernel(TopkOutputParams p)
 {
@@ -248,6 +426,8 @@ __global__ void __launch_bounds__(kTopkThreads) topk_output_kernel(TopkOutputPar
   constexpr int kCap       = S::kCap;
   constexpr int kIpt       = S::kIpt;
   constexpr uint32_t kFull = uint32_t(kCap / 2); // flush above this many items
+  constexpr int kBits      = int(sizeof(UK) * 8);
+  constexpr int kTop       = kBits - kRadixBits;
   __shared__ UK s_klt[kCap];
   __shared__ UK s_keq[kCap];
   __shared__ typename S::VS s_vlt[S::kValCap];
@@ -256,14 +436,17 @@ __global__ void __launch_bounds__(kTopkThreads) topk_output_kernel(TopkOutputPar
   __shared__ uint64_t s_base;
   __shared__ uint32_t s_eq_full;
 
-  TopkState* st         = p.st;
-  const int s           = st->done != 0u ? int(st->done_shift) : 0;
-  const UK thr          = UK(UK(st->prefix) >> s);
-  const uint64_t k_rem  = st->k_rem;
-  const uint64_t eq0    = p.k - k_rem; // keys below the threshold fill [0, eq0), equal ones [eq0, k)
-  const UK* keys        = static_cast<const UK*>(p.keys_in);
-  UK* keys_out          = static_cast<UK*>(p.keys_out);
-  const bool select_max = p.select_max != 0;
+  TopkState* st             = p.st;
+  const int s               = st->done != 0u ? int(st->done_shift) : 0;
+  const UK thr              = UK(UK(st->prefix) >> s);
+  const bool filtered       = st->filtered != 0u;
+  const UK d0               = UK(UK(st->prefix) >> kTop);
+  const uint64_t k_rem      = st->k_rem;
+  const uint64_t eq0        = p.k - k_rem; // keys below the threshold fill [0, eq0), equal ones [eq0, k)
+  const TopkSourceView view = topk_source(st, p.src);
+  const UK* keys            = static_cast<const UK*>(view.cand ? p.src.cand_keys : p.src.keys);
+  UK* keys_out              = static_cast<UK*>(p.keys_out);
+  const bool select_max     = p.select_max != 0;
   if (threadIdx.x == 0)
   {
     s_n[0] = s_n[1] = 0u;
@@ -272,47 +455,52 @@ __global__ void __launch_bounds__(kTopkThreads) topk_output_kernel(TopkOutputPar
   __syncthreads();
 
   const uint64_t chunk = uint64_t(kTopkThreads) * kIpt;
-  for (uint64_t base = uint64_t(blockIdx.x) * chunk; base < p.num_items; base += uint64_t(gridDim.x) * chunk)
+  for (uint64_t base = uint64_t(blockIdx.x) * chunk; base < view.count; base += uint64_t(gridDim.x) * chunk)
   {
     UK k[kIpt];
 #pragma unroll
     for (int j = 0; j < kIpt; ++j)
     {
       const uint64_t i = base + uint64_t(j) * kTopkThreads + threadIdx.x;
-      k[j]             = i < p.num_items ? keys[i] : UK(0);
+      k[j]             = i < view.count ? keys[i] : UK(0);
     }
     const bool take_eq = s_eq_full == 0u;
 #pragma unroll
     for (int j = 0; j < kIpt; ++j)
     {
       const uint64_t i   = base + uint64_t(j) * kTopkThreads + threadIdx.x;
-      const bool in      = i < p.num_items;
-      const UK o         = UK(topk_order(k[j], p.key_kind, select_max) >> s);
+      const UK full      = topk_order(k[j], p.key_kind, select_max);
+      const bool in      = i < view.count && (!filtered || UK(full >> kTop) == d0);
+      const UK o         = UK(full >> s);
       const bool lt      = in && o < thr;
       const bool eq      = in && take_eq && o == thr;
       const uint64_t mlt = __ballot(lt);
       const uint64_t meq = __ballot(eq);
-      if (lt)
+      if (lt || eq)
       {
-        const uint32_t slot = wave_lds_slot(&s_n[0], mlt);
-        s_klt[slot]         = k[j];
-        if constexpr (S::kHasValue)
+        const uint64_t src_i = view.cand ? uint64_t(p.src.cand_idx[i]) : i;
+        if (lt)
         {
-          s_vlt[slot] = static_cast<const V*>(p.vals_in)[i];
+          const uint32_t slot = wave_lds_slot(&s_n[0], mlt);
+          s_klt[slot]         = k[j];
+          if constexpr (S::kHasValue)
+          {
+            s_vlt[slot] = static_cast<const V*>(p.vals_in)[src_i];
+          }
         }
-      }
-      if (eq)
-      {
-        const uint32_t slot = wave_lds_slot(&s_n[1], meq);
-        s_keq[slot]         = k[j];
-        if constexpr (S::kHasValue)
+        else
         {
-          s_veq[slot] = static_cast<const V*>(p.vals_in)[i];
+          const uint32_t slot = wave_lds_slot(&s_n[1], meq);
+          s_keq[slot]         = k[j];
+          if constexpr (S::kHasValue)
+          {
+            s_veq[slot] = static_cast<const V*>(p.vals_in)[src_i];
+          }
         }
       }
     }
     __syncthreads();
-    const bool last = base + uint64_t(gridDim.x) * chunk >= p.num_items;
+    const bool last    = base + uint64_t(gridDim.x) * chunk >= view.count;
     const uint32_t nlt = s_n[0];
     const uint32_t neq = s_n[1];
     if (nlt > kFull || (last && nlt > 0u))
@@ -351,49 +539,62 @@ __global__ void __launch_bounds__(kTopkThreads) topk_output_kernel(TopkOutputPar
   }
 }
 
-template <class UK>
-const void* output_kernel_for(int val_bytes)
-{
-  switch (val_bytes)
-  {
-    case 0:
-      return reinterpret_cast<const void*>(&topk_output_kernel<UK, NoValue>);
-    case 1:
-      return reinterpret_cast<const void*>(&topk_output_kernel<UK, uint8_t>);
-    case 2:
-      return reinterpret_cast<const void*>(&topk_output_kernel<UK, uint16_t>);
-    case 4:
-      return reinterpret_cast<const void*>(&topk_output_kernel<UK, uint32_t>);
-    case 8:
-      return reinterpret_cast<const void*>(&topk_output_kernel<UK, uint64_t>);
-    case 16:
-      return reinterpret_cast<const void*>(&topk_output_kernel<UK, Value16>);
-    default:
-      return nullptr;
-  }
-}
-
 struct TopkKernels
 {
   const void* hist;
+  const void* filter;
   const void* output;
 };
+
+template <class UK>
+TopkKernels kernels_for_value(int val_bytes)
+{
+  auto make = [](auto v) -> TopkKernels {
+    using V = decltype(v);
+    return {reinterpret_cast<const void*>(&topk_hist_kernel<UK>),
+            reinterpret_cast<const void*>(&topk_filter_kernel<UK, V>),
+            reinterpret_cast<const void*>(&topk_output_kernel<UK, V>)};
+  };
+  switch (val_bytes)
+  {
+    case 0:
+      return make(NoValue{});
+    case 1:
+      return make(uint8_t{});
+    case 2:
+      return make(uint16_t{});
+    case 4:
+      return make(uint32_t{});
+    case 8:
+      return make(uint64_t{});
+    case 16:
+      return make(Value16{});
+    default:
+      return {nullptr, nullptr, nullptr};
+  }
+}
 
 TopkKernels topk_kernels(int key_bytes, int val_bytes)
 {
   switch (key_bytes)
   {
     case 1:
-      return {reinterpret_cast<const void*>(&topk_hist_kernel<uint8_t>), output_kernel_for<uint8_t>(val_bytes)};
+      return kernels_for_value<uint8_t>(val_bytes);
     case 2:
-      return {reinterpret_cast<const void*>(&topk_hist_kernel<uint16_t>), output_kernel_for<uint16_t>(val_bytes)};
+      return kernels_for_value<uint16_t>(val_bytes);
     case 4:
-      return {reinterpret_cast<const void*>(&topk_hist_kernel<uint32_t>), output_kernel_for<uint32_t>(val_bytes)};
+      return kernels_for_value<uint32_t>(val_bytes);
     case 8:
-      return {reinterpret_cast<const void*>(&topk_hist_kernel<uint64_t>), output_kernel_for<uint64_t>(val_bytes)};
+      return kernels_for_value<uint64_t>(val_bytes);
     default:
-      return {nullptr, nullptr};
+      return {nullptr, nullptr, nullptr};
   }
+}
+
+// Candidate-buffer capacity: a sixteenth of the input (at least 2^16 items, at most all).
+uint64_t topk_cand_cap(uint64_t num_items)
+{
+  return std::min<uint64_t>(num_items, std::max<uint64_t>(num_items / 16, uint64_t(1) << 16));
 }
 
 } // namespace
@@ -414,19 +615,21 @@ hipError_t topk(
   hipStream_t stream)
 {
   const TopkKernels kern = topk_kernels(key_bytes, val_bytes);
-  if (kern.hist == nullptr || kern.output == nullptr)
+  if (kern.hist == nullptr)
   {
     set_last_error("top-k: unsupported key width %d or value width %d", key_bytes, val_bytes);
     return hipErrorInvalidValue;
   }
   if (num_items > 0xffffffffull)
   {
-    set_last_error("top-k: at most 2^32 - 1 items (bins and slots are 32-bit counters)");
+    set_last_error("top-k: at most 2^32 - 1 items (bins, slots and indices are 32-bit)");
     return hipErrorInvalidValue;
   }
-  size_t sizes[1] = {sizeof(TopkState)};
-  void* parts[1]  = {};
-  if (hipError_t e = alias_temporaries(d_temp_storage, temp_storage_bytes, parts, sizes, 1); e != hipSuccess)
+  const bool filter = key_bytes > 1; // one-byte keys have a single digit pass
+  const uint64_t cap = filter ? topk_cand_cap(num_items) : 0;
+  size_t sizes[3]    = {sizeof(TopkState), size_t(cap) * key_bytes, size_t(cap) * sizeof(uint32_t)};
+  void* parts[3]     = {};
+  if (hipError_t e = alias_temporaries(d_temp_storage, temp_storage_bytes, parts, sizes, 3); e != hipSuccess)
   {
     return e;
   }
@@ -451,6 +654,7 @@ hipError_t topk(
     return e;
   }
   TopkState* st = static_cast<TopkState*>(parts[0]);
+  const TopkSource src{d_keys_in, parts[1], static_cast<const uint32_t*>(parts[2]), num_items, uint32_t(cap)};
   {
     TopkInitParams ip{st, k};
     if (hipError_t e = launch_kernel(reinterpret_cast<const void*>(&topk_init_kernel), 1, kTopkThreads, 0, stream,
@@ -462,10 +666,21 @@ hipError_t topk(
   }
   const uint64_t chunks = (num_items + uint64_t(kTopkThreads) * kTopkIpt - 1) / (uint64_t(kTopkThreads) * kTopkIpt);
   const uint64_t grid   = std::min<uint64_t>(chunks, uint64_t(device_cu_count()) * 4);
+  const int sm          = select_max ? 1 : 0;
   for (int pass = 0; pass < key_bytes; ++pass)
   {
-    TopkPassParams pp{d_keys_in, num_items, st, key_kind, select_max ? 1 : 0, (key_bytes - 1 - pass) * kRadixBits};
-    if (hipError_t e = launch_kernel(kern.hist, grid, kTopkThreads, 0, stream, &pp, 11); e != hipSuccess)
+    const int shift = (key_bytes - 1 - pass) * kRadixBits;
+    TopkPassParams pp{src, st, key_kind, sm, shift};
+    if (pass == 1)
+    {
+      TopkFilterParams fp{src, d_values_in, d_keys_out, d_values_out, k, st, parts[1], static_cast<uint32_t*>(parts[2]),
+                          key_kind, sm};
+      if (hipError_t e = launch_kernel(kern.filter, grid, kTopkThreads, 0, stream, &fp, 11); e != hipSuccess)
+      {
+        return e;
+      }
+    }
+    else if (hipError_t e = launch_kernel(kern.hist, grid, kTopkThreads, 0, stream, &pp, 11); e != hipSuccess)
     {
       return e;
     }
@@ -476,7 +691,7 @@ hipError_t topk(
       return e;
     }
   }
-  TopkOutputParams op{d_keys_in, d_values_in, d_keys_out, d_values_out, num_items, k, st, key_kind, select_max ? 1 : 0};
+  TopkOutputParams op{src, d_values_in, d_keys_out, d_values_out, k, st, key_kind, sm};
   return launch_kernel(kern.output, grid, kTopkThreads, 0, stream, &op, 11);
 }
 

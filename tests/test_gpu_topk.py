@@ -145,6 +145,32 @@ def test_selection_completes_early():
         check(keys, k, False)
 
 
+def test_selection_completes_in_the_filter_pass():
+    """The second digit decides: once with the candidates (keys on the top digit) fitting
+    the candidate buffer, once with every key a candidate (the buffer overflows and the
+    output pass reads the input again)."""
+    rng = np.random.default_rng(2)
+    a = np.repeat(np.arange(256, dtype=np.uint32), 16 * 256)
+    b = np.tile(np.repeat(np.arange(256, dtype=np.uint32), 16), 256)
+    spread = rng.permutation((a << np.uint32(24)) | (b << np.uint32(16)) | np.uint32(77))
+    for k in (16 * 256 * 3 + 16 * 5, 16 * 7):
+        check(spread, k, True)
+        check(spread, k, False)
+    one_top = rng.permutation((b << np.uint32(16)) | np.uint32(5))  # every key on top digit 0
+    for k in (16 * 9, 16 * 200 + 3):
+        check(one_top, k, True)
+        check(one_top, k, False)
+
+
+def test_candidate_overflow_falls_back_to_the_input():
+    """Min over AND-5 keys: most keys share the top digit 0, far more than the candidate
+    buffer (a sixteenth of the input) holds."""
+    rng = np.random.default_rng(8)
+    keys = random_keys(rng, np.uint32, 1 << 20, "and5")
+    for k in (1, 1000, 300_000):
+        check(keys, k, False)
+
+
 def test_capped_k_zero_k_and_all_equal():
     rng = np.random.default_rng(4)
     keys = gen(rng, np.uint32, 5000)

@@ -101,7 +101,11 @@ def test_entry_point_query_and_rejections():
         for vb in (0, 1, 4, 16):
             rc, nbytes = _query(kt, vb, 1 << 20, 100)
             assert rc == 0 and nbytes > 0
-    assert _query(B.CCCL_UINT32, 0, 0, 0) == (0, _query(B.CCCL_UINT32, 0, 10, 3)[1])
+    assert _query(B.CCCL_UINT32, 0, 0, 0)[0] == 0 and _query(B.CCCL_UINT32, 0, 0, 0)[1] > 0
+    # the candidate buffer of the filter pass: keys + 4-byte indices for a sixteenth of the input
+    n = 1 << 24
+    assert _query(B.CCCL_UINT32, 4, n, 3)[1] >= (n // 16) * (4 + 4)
+    assert _query(B.CCCL_UINT8, 4, n, 3)[1] < 4096  # one digit pass: no filter, no buffer
     assert _query(B.CCCL_UINT32, 3, 100, 3)[0] == 801  # value width
     assert _query(B.CCCL_STORAGE, 0, 100, 3)[0] == 801  # key type
     assert _query(B.CCCL_UINT32, 0, 1 << 32, 3)[0] == 1  # 32-bit counters
