@@ -131,7 +131,15 @@ __device__ __forceinline__ void upsweep_block(const UpsweepParams& p, uint32_t* 
         for (int u = 0; u < kU; ++u)
         {
           const uint32_t v = base + uint32_t(u * kThreads) + tid;
-          x[u]             = v < nvec ? src[v] : uint4{0u, 0u, 0u, 0u};
+          // Non-temporal: the keys are read once per pass (0.24 -> 0.22 ms per pass at
+          // 2^28 u32 keys); the downsweep's loads stay temporal (it was slower with them).
+          using NV         = unsigned int __attribute__((ext_vector_type(4)));
+          NV t             = NV{0u, 0u, 0u, 0u};
+          if (v < nvec)
+          {
+            t = __builtin_nontemporal_load(reinterpret_cast<const NV*>(src) + v);
+          }
+          __builtin_memcpy(&x[u], &t, 16);
         }
       };
       auto count_batch = [&](uint32_t base, const uint4(&x)[kU]) {
@@ -1666,7 +1674,7 @@ __device__ __forceinline__ void seg_upsweep_tile(const SegUpsweepParams& p, uint
     for (int j = 0; j < kBatch; ++j)
     {
       const uint32_t i = uint32_t(j0 + j) * kThreads + tid;
-      k[j]             = i < valid ? keys[i] : UK(0);
+      k[j]             = i < valid ? __builtin_nontemporal_load(&keys[i]) : UK(0); // read once per pass
     }
 #pragma unroll
     for (int j = 0; j < kBatch; ++j)

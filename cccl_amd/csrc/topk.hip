@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kTopkThreads) topk_hist_kernel(TopkPassParams 
     for (int j = 0; j < kTopkIpt; ++j)
     {
       const uint64_t i = base + uint64_t(j) * kTopkThreads + threadIdx.x;
-      k[j]             = i < view.count ? keys[i] : UK(0);
+      k[j]             = i < view.count ? __builtin_nontemporal_load(&keys[i]) : UK(0); // streamed once
     }
 #pragma unroll
     for (int j = 0; j < kTopkIpt; ++j)
@@ -319,7 +319,7 @@ __global__ void __launch_bounds__(kTopkThreads) topk_filter_kernel(TopkFilterPar
     for (int j = 0; j < kTopkIpt; ++j)
     {
       const uint64_t i = base + uint64_t(j) * kTopkThreads + threadIdx.x;
-      k[j]             = i < n ? keys[i] : UK(0);
+      k[j]             = i < n ? __builtin_nontemporal_load(&keys[i]) : UK(0); // streamed once
     }
     const bool last_step = base + uint64_t(gridDim.x) * chunk >= n;
 #pragma unroll
