@@ -565,7 +565,16 @@ __device__ __forceinline__ void onesweep_tile(const OnesweepParams& p, unsigned 
 #pragma unroll
       for (int i = 0; i < IPT; ++i)
       {
-        vals[i] = vin[i * 64];
+        // Non-temporal for the values (read once, after the rank): 0.93 -> 0.92 ms per
+        // 2^28-pair pass; the same hint on the keys made the pass slower.
+        if constexpr (std::is_integral<VS>::value)
+        {
+          vals[i] = __builtin_nontemporal_load(&vin[i * 64]);
+        }
+        else
+        {
+          vals[i] = vin[i * 64];
+        }
       }
     }
     else
