@@ -333,16 +333,21 @@ cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64
   auto pow2 = [](int e) { return uint64_t(1) << e; };
   if (n >= pow2(22))
   {
-    // 32-bit keys with 32-bit values from 2^26 items: the largest tile (longest digit
+    // 32-bit keys with 32-bit values from 2^25 items: the largest tile (longest digit
     // runs per tile, so the scatter writes fewer partial lines); pairs of 12 to 16 bytes:
     // 1024 x 8; otherwise 512 x 16 (512 x 8 below 2^24 items for 4-byte keys), where two
     // workgroups per CU hide the rank's latency better; wider pairs: 256 x 16.
-    if (key_bytes == 4 && val_bytes == 4 && n >= pow2(26) && pick(1024, 16, CCCL_AMD_ALGO_PASSES))
+    if (key_bytes == 4 && val_bytes == 4 && n >= pow2(25) && pick(1024, 16, CCCL_AMD_ALGO_PASSES))
     {
       return c;
     }
-    // 8-byte keys with 4-byte values below 2^30 items: the onesweep with look-back
-    // (1024 x 16, values staged after the keys), 3-6% ahead of the passes at 2^27.
+    // 8-byte keys with 4-byte values: from 2^27 items the passes with 1024 x 16 tiles
+    // (values staged after the keys; 2^27 pairs 6.44 vs 6.65 ms with look-back), below
+    // that the onesweep with look-back of the same shape (2% ahead at 2^24-2^26).
+    if (key_bytes == 8 && val_bytes == 4 && n >= pow2(27) && pick(1024, 16, CCCL_AMD_ALGO_PASSES))
+    {
+      return c;
+    }
     if (key_bytes == 8 && val_bytes == 4 && n < pow2(30) && pick(1024, 16, CCCL_AMD_ALGO_ONESWEEP))
     {
       return c;
