@@ -385,6 +385,72 @@ void thrust_pairs_case()
   report("thrust_compat::stable_sort_by_key f64 greater, own stream", same_bits(dk.host(n), ek) && dv.host(n) == ev);
 }
 
+void segmented_keys_doublebuffer_case()
+{
+  const size_t nseg = 150;
+  std::vector<int64_t> beg(nseg), end(nseg);
+  uint64_t sd = 31;
+  size_t n    = 0;
+  for (size_t i = 0; i < nseg; ++i)
+  {
+    const uint64_t r = splitmix(sd);
+    beg[i]           = int64_t(n);
+    n += (r % 7 == 0) ? 30000 + r % 20000 : r % 900;
+    end[i] = int64_t(n);
+  }
+  auto keys = random_keys<int16_t>(n, 32);
+  Dev<int16_t> k0(keys), k1(n);
+  Dev<int64_t> db(beg), de(end);
+  cccl_amd::DoubleBuffer<int16_t> dk(k0.p, k1.p);
+  size_t bytes = 0;
+  HIP_CHECK(cccl_amd::DeviceSegmentedRadixSort::SortKeysDescending(nullptr, bytes, dk, n, nseg, db.p, de.p, 3, 14));
+  Temp t(bytes);
+  HIP_CHECK(cccl_amd::DeviceSegmentedRadixSort::SortKeysDescending(t.p, bytes, dk, n, nseg, db.p, de.p, 3, 14));
+  std::vector<uint32_t> perm(n);
+  std::iota(perm.begin(), perm.end(), 0u);
+  for (size_t i = 0; i < nseg; ++i)
+  {
+    stable_perm(keys, size_t(beg[i]), size_t(end[i]), true, 3, 14, perm);
+  }
+  std::vector<int16_t> ek(n), hk(n);
+  for (size_t i = 0; i < n; ++i)
+  {
+    ek[i] = keys[perm[i]];
+  }
+  HIP_CHECK(hipMemcpy(hk.data(), dk.Current(), n * sizeof(int16_t), hipMemcpyDeviceToHost));
+  report("DeviceSegmentedRadixSort::SortKeysDescending i16 DoubleBuffer bits [3,14)", same_bits(hk, ek));
+}
+
+void topk_min_keys_case()
+{
+  const size_t n = 3 << 20, k = 100000;
+  auto keys      = random_keys<double>(n, 33);
+  Dev<double> kin(keys), kout(k);
+  size_t bytes = 0;
+  HIP_CHECK(cccl_amd::DeviceTopK::MinKeys(nullptr, bytes, kin.p, kout.p, n, k));
+  Temp t(bytes);
+  HIP_CHECK(cccl_amd::DeviceTopK::MinKeys(t.p, bytes, kin.p, kout.p, n, k));
+  auto got = kout.host(k);
+  // the k smallest by the radix image (+0.0 above -0.0), as a multiset of images
+  std::vector<uint64_t> img(n), out(k);
+  auto image = [](double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, 8);
+    return (u >> 63) ? ~u : (u | (uint64_t(1) << 63));
+  };
+  for (size_t i = 0; i < n; ++i)
+  {
+    img[i] = image(keys[i]);
+  }
+  for (size_t i = 0; i < k; ++i)
+  {
+    out[i] = image(got[i]);
+  }
+  std::sort(img.begin(), img.end());
+  std::sort(out.begin(), out.end());
+  report("DeviceTopK::MinKeys f64 (signed zeros, ties)", std::equal(out.begin(), out.end(), img.begin()));
+}
+
 bool query_only()
 {
   size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
@@ -422,6 +488,8 @@ int main(int argc, char** argv)
     thrust_keys_case<uint32_t>("thrust_compat::sort u32 less", (1 << 20) + 5, 22, std::less<uint32_t>());
     thrust_keys_case<uint8_t>("thrust_compat::sort u8 greater<> (copy back)", 100000, 23, std::greater<>());
     thrust_pairs_case();
+    segmented_keys_doublebuffer_case();
+    topk_min_keys_case();
   }
   std::printf("%s: %d failure(s)\n", g_failures ? "FAILED" : "OK", g_failures);
   return g_failures ? 1 : 0;

@@ -30,4 +30,4 @@ def test_cpp_api_size_queries_without_a_device():
 def test_cpp_api_on_the_gpu():
     r = _run()
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("PASS") == 14 and "FAIL" not in r.stdout
+    assert r.stdout.count("PASS") == 16 and "FAIL" not in r.stdout
