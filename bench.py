@@ -539,7 +539,30 @@ def main() -> int:
     k1o = torch.empty_like(k1)
     s1 = ca.Sorter(torch.uint32, 0, n1)
     ms1 = time_steps(torch, lambda: s1.sort(k1, k1o), 50, 5)
-    del s1, k1, k1o
+    # the same sort captured once in a hipGraph and replayed (a latency-bound caller's form)
+    from cccl_amd.graph import SortGraph
+
+    side = torch.cuda.Stream()
+    g1 = SortGraph()
+    with torch.cuda.stream(side):
+        with g1.capture(side):
+            s1.sort(k1, k1o, stream=side)
+    side.synchronize()
+    ref1 = k1o.clone()
+    k1o.zero_()
+    torch.cuda.synchronize()
+    for _ in range(5):
+        g1.launch(side)
+    side.synchronize()
+    a1, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a1.record(side)
+    for _ in range(50):
+        g1.launch(side)
+    b1.record(side)
+    side.synchronize()
+    ms1_graph = a1.elapsed_time(b1) / 50
+    graph1_ok = bool(torch.equal(k1o, ref1))
+    del s1, k1, k1o, g1, ref1
 
     extra = {}
     if not args.quick:
@@ -588,6 +611,7 @@ def main() -> int:
         "gkeys_per_s_by_entropy": rates,
         "parity_properties_ok": parity,
         "sortkeys_1m_u32_us": round(ms1 * 1e3, 2),
+        "sortkeys_1m_u32_hipgraph_us": round(ms1_graph * 1e3, 2), "sortkeys_1m_u32_hipgraph_ok": graph1_ok,
         "sortkeys_1m_u32_gkeys_per_s": round(n1 / (ms1 * 1e-3) / 1e9, 3),
         **extra,
     }
