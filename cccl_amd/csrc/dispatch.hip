@@ -335,8 +335,9 @@ cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64
   {
     // 32-bit keys with 32-bit values from 2^25 items: the largest tile (longest digit
     // runs per tile, so the scatter writes fewer partial lines); pairs of 12 to 16 bytes:
-    // 1024 x 8; otherwise 512 x 16 (512 x 8 below 2^24 items for 4-byte keys), where two
-    // workgroups per CU hide the rank's latency better; wider pairs: 256 x 16.
+    // 1024 x 8; otherwise 512 x 16, where two workgroups per CU hide the rank's latency
+    // better (for 4-byte keys 2-12% ahead of 512 x 8 from 2^22 items); wider pairs:
+    // 256 x 16.
     if (key_bytes == 4 && val_bytes == 4 && n >= pow2(25) && pick(1024, 16, CCCL_AMD_ALGO_PASSES))
     {
       return c;
@@ -353,10 +354,6 @@ cccl_amd_radix_sort_config_t default_config(int key_bytes, int val_bytes, uint64
       return c;
     }
     if (val_bytes > 0 && item_bytes >= 12 && item_bytes <= 16 && pick(1024, 8, CCCL_AMD_ALGO_PASSES))
-    {
-      return c;
-    }
-    if (key_bytes == 4 && n < pow2(24) && pick(512, 8, CCCL_AMD_ALGO_PASSES))
     {
       return c;
     }
